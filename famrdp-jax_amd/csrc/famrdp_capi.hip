@@ -1,0 +1,559 @@
+// C-ABI layer: argument checks, plans (block table, halo gather table, BC
+// tables, tile table, stage workspace), launches, hipGraph capture and
+// error reporting.  Declarations: include/famrdp_capi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "famrdp_kernels.hip"
+#include "../../include/famrdp_capi.h"
+
+using namespace famrdp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(FAMRDP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH_CHECK()                                                                         \
+  do {                                                                                         \
+    hipError_t e_ = hipGetLastError();                                                         \
+    if (e_ != hipSuccess) return fail(FAMRDP_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+Gas to_gas(const famrdp_gas* g) { return Gas{g->gamma, g->R, g->mu0, g->T0, g->Ts, g->Pr_lam}; }
+
+bool bad_lim(int32_t lim) { return lim != FAMRDP_LIMITER_MINMOD && lim != FAMRDP_LIMITER_VANLEER; }
+
+}  // namespace
+
+struct famrdp_plan {
+  int nblocks = 0, ghost = 2, limiter = 0;
+  Gas gas{};
+  std::vector<int32_t> dims;             // 3 per block
+  std::vector<int64_t> N;
+  // device tables
+  BlockDesc* d_blocks = nullptr;
+  BlockDesc* h_blocks_pinned = nullptr;  // staging for table uploads
+  std::vector<BlockDesc> h_blocks;
+  bool blocks_uploaded = false;
+  HaloEntry* d_halo = nullptr;
+  int nhalo = 0;
+  double* d_halo_stage = nullptr;
+  BcEntry* d_bc[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int nbc[6] = {0, 0, 0, 0, 0, 0};
+  int64_t bc_maxcells[6] = {0, 0, 0, 0, 0, 0};
+  Tile* d_tiles = nullptr;
+  int ntiles = 0;
+  int persistent_grid = 0;
+  // workspace
+  std::vector<double*> q1, q2, G;
+  // graph cache
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<const void*> graph_key;
+  double graph_dt = 0.0;
+  int graph_flags = -1;
+  bool has_workspace = true;
+  // stage timing (HIP events on the launch stream, eager launches only)
+  bool timing = false;
+  bool capturing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+// Face geometry: axis (0..2) and low/high side.
+inline int face_axis(int f) { return f / 2; }
+inline bool face_lo(int f) { return f % 2 == 0; }
+
+int build_halo_table(famrdp_plan* p, int ncut, const famrdp_cut* cuts, std::vector<HaloEntry>& out) {
+  const int g = p->ghost;
+  // final writer per target cell: slot[b][cell] = index into `src` (or -1)
+  std::vector<std::vector<int64_t>> slot(p->nblocks);
+  std::vector<std::pair<int32_t, int32_t>> src;
+  for (int c = 0; c < ncut; ++c) {
+    const famrdp_cut& ct = cuts[c];
+    if (ct.tgt_block < 0 || ct.tgt_block >= p->nblocks || ct.src_block < 0 || ct.src_block >= p->nblocks ||
+        ct.tgt_face < 0 || ct.tgt_face > 5 || ct.src_face < 0 || ct.src_face > 5)
+      return fail(FAMRDP_E_INVALID, "cut " + std::to_string(c) + ": block or face out of range");
+    if (ct.orient != 0)
+      return fail(FAMRDP_E_UNSUPPORTED, "orient=" + std::to_string(ct.orient) + " not supported in stage 1");
+    const int32_t* dt = &p->dims[3 * ct.tgt_block];
+    const int32_t* ds = &p->dims[3 * ct.src_block];
+    int lo_t[3] = {0, 0, 0}, ext_t[3] = {dt[0], dt[1], dt[2]};
+    int lo_s[3] = {0, 0, 0}, ext_s[3] = {ds[0], ds[1], ds[2]};
+    const int at = face_axis(ct.tgt_face), as = face_axis(ct.src_face);
+    lo_t[at] = face_lo(ct.tgt_face) ? 0 : dt[at] - g;  // halo.py:19-23
+    ext_t[at] = g;
+    lo_s[as] = face_lo(ct.src_face) ? g : ds[as] - 2 * g;  // halo.py:26-33
+    ext_s[as] = g;
+    if (ext_t[0] != ext_s[0] || ext_t[1] != ext_s[1] || ext_t[2] != ext_s[2])
+      return fail(FAMRDP_E_INVALID, "cut " + std::to_string(c) + ": ghost slab and source slab shapes differ");
+    auto& sl = slot[ct.tgt_block];
+    if (sl.empty()) sl.assign((size_t)p->N[ct.tgt_block], -1);
+    for (int a = 0; a < ext_t[0]; ++a)
+      for (int b = 0; b < ext_t[1]; ++b)
+        for (int k = 0; k < ext_t[2]; ++k) {
+          const int64_t tc = ((int64_t)(lo_t[0] + a) * dt[1] + (lo_t[1] + b)) * dt[2] + (lo_t[2] + k);
+          const int64_t sc = ((int64_t)(lo_s[0] + a) * ds[1] + (lo_s[1] + b)) * ds[2] + (lo_s[2] + k);
+          sl[(size_t)tc] = (int64_t)src.size();
+          src.emplace_back(ct.src_block, (int32_t)sc);
+        }
+  }
+  for (int b = 0; b < p->nblocks; ++b) {
+    if (slot[b].empty()) continue;
+    for (int64_t tc = 0; tc < p->N[b]; ++tc) {
+      const int64_t s = slot[b][(size_t)tc];
+      if (s < 0) continue;
+      out.push_back(HaloEntry{b, (int32_t)tc, src[(size_t)s].first, src[(size_t)s].second});
+    }
+  }
+  return FAMRDP_OK;
+}
+
+int bind(famrdp_plan* p, double* const* q, const double* const* kxyz, const double* const* vol, hipStream_t st) {
+  if (!q) return fail(FAMRDP_E_INVALID, "q pointer array is NULL");
+  bool changed = !p->blocks_uploaded;
+  for (int b = 0; b < p->nblocks; ++b) {
+    BlockDesc& d = p->h_blocks[b];
+    if (!q[b]) return fail(FAMRDP_E_INVALID, "q[" + std::to_string(b) + "] is NULL");
+    const double* kx = kxyz ? kxyz[b] : d.kxyz;
+    const double* vo = vol ? vol[b] : d.vol;
+    if (d.q != q[b] || d.kxyz != kx || d.vol != vo) changed = true;
+    d.q = q[b];
+    d.kxyz = kx;
+    d.vol = vo;
+  }
+  if (changed) {
+    // the previous upload (if any) has been consumed in stream order
+    HIP_TRY(hipStreamSynchronize(st));
+    std::memcpy(p->h_blocks_pinned, p->h_blocks.data(), sizeof(BlockDesc) * p->nblocks);
+    HIP_TRY(hipMemcpyAsync(p->d_blocks, p->h_blocks_pinned, sizeof(BlockDesc) * p->nblocks, hipMemcpyHostToDevice, st));
+    p->blocks_uploaded = true;
+  }
+  return FAMRDP_OK;
+}
+
+int launch_halo(famrdp_plan* p, hipStream_t st) {
+  if (p->nhalo == 0) return FAMRDP_OK;
+  k_halo_gather<<<nblk(p->nhalo), 256, 0, st>>>(p->d_blocks, p->d_halo, p->nhalo, p->d_halo_stage);
+  LAUNCH_CHECK();
+  k_halo_scatter<<<nblk(p->nhalo), 256, 0, st>>>(p->d_blocks, p->d_halo, p->nhalo, p->d_halo_stage);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int launch_bc(famrdp_plan* p, hipStream_t st) {
+  for (int f = 0; f < 6; ++f) {
+    if (p->nbc[f] == 0) continue;
+    dim3 grid(nblk(p->bc_maxcells[f]), (unsigned)p->nbc[f]);
+    k_bc_face<<<grid, 256, 0, st>>>(p->d_blocks, p->d_bc[f], p->ghost, p->gas.gamma, p->gas.R);
+    LAUNCH_CHECK();
+  }
+  return FAMRDP_OK;
+}
+
+hipEvent_t next_event(famrdp_plan* p) {
+  if (p->ev_used == p->ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    p->ev_pool.push_back(e);
+  }
+  return p->ev_pool[p->ev_used++];
+}
+
+int launch_stages(famrdp_plan* p, double dt, int flags, hipStream_t st) {
+  for (int stage = 1; stage <= 3; ++stage) {
+    const bool tm = p->timing && !p->capturing;
+    hipEvent_t e0 = tm ? next_event(p) : nullptr, e1 = tm ? next_event(p) : nullptr;
+    if (tm && (!e0 || !e1)) return fail(FAMRDP_E_HIP, "hipEventCreate failed");
+    if (tm) HIP_TRY(hipEventRecord(e0, st));
+    if (flags & FAMRDP_STEP_BATCHED) {
+      k_viscous_flux_batched<<<p->persistent_grid, 256, 0, st>>>(p->d_blocks, p->d_tiles, p->ntiles, stage, p->gas);
+      LAUNCH_CHECK();
+      k_stage_batched<<<p->persistent_grid, 256, 0, st>>>(p->d_blocks, p->d_tiles, p->ntiles, p->gas.gamma,
+                                                          p->limiter, dt, stage);
+      LAUNCH_CHECK();
+    } else {
+      for (int b = 0; b < p->nblocks; ++b) {
+        const BlockDesc& d = p->h_blocks[b];
+        const double* qin = stage == 1 ? d.q : (stage == 2 ? d.q1 : d.q2);
+        double* qout = stage == 1 ? d.q1 : (stage == 2 ? d.q2 : d.q);
+        k_viscous_flux<<<nblk(d.N), 256, 0, st>>>(qin, d.kxyz, d.G, d.ni, d.nj, d.nk, p->gas);
+        LAUNCH_CHECK();
+        k_stage<<<nblk(d.N), 256, 0, st>>>(qin, d.q, qout, d.kxyz, d.vol, d.G, d.ni, d.nj, d.nk, p->gas.gamma,
+                                           p->limiter, dt, stage);
+        LAUNCH_CHECK();
+      }
+    }
+    if (tm) HIP_TRY(hipEventRecord(e1, st));
+  }
+  return FAMRDP_OK;
+}
+
+int run_step(famrdp_plan* p, double dt, int flags, hipStream_t st) {
+  int rc = launch_halo(p, st);
+  if (rc) return rc;
+  rc = launch_bc(p, st);
+  if (rc) return rc;
+  return launch_stages(p, dt, flags, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* famrdp_last_error(void) { return g_last_error.c_str(); }
+
+const char* famrdp_version(void) { return "famrdp-mi355x 0.1 (gfx950, fp64, direct stage kernels)"; }
+
+int famrdp_cons_to_prim(const double* q, double* pv, int64_t n, double gamma, void* stream) {
+  if (n < 0 || (n > 0 && (!q || !pv))) return fail(FAMRDP_E_INVALID, "cons_to_prim: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_cons_to_prim<<<nblk(n), 256, 0, S(stream)>>>(q, pv, n, gamma);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_prim_to_cons(const double* pv, double* q, int64_t n, double gamma, void* stream) {
+  if (n < 0 || (n > 0 && (!q || !pv))) return fail(FAMRDP_E_INVALID, "prim_to_cons: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_prim_to_cons<<<nblk(n), 256, 0, S(stream)>>>(pv, q, n, gamma);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_sutherland(const double* T, double* mu, int64_t n, double mu0, double T0, double Ts, void* stream) {
+  if (n < 0 || (n > 0 && (!T || !mu))) return fail(FAMRDP_E_INVALID, "sutherland: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_sutherland<<<nblk(n), 256, 0, S(stream)>>>(T, mu, n, mu0, T0, Ts);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_speed_of_sound(const double* rho, const double* p, double* c, int64_t n, double gamma, void* stream) {
+  if (n < 0 || (n > 0 && (!rho || !p || !c))) return fail(FAMRDP_E_INVALID, "speed_of_sound: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_speed_of_sound<<<nblk(n), 256, 0, S(stream)>>>(rho, p, c, n, gamma);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_limiter(const double* a, const double* b, double* out, int64_t n, int32_t limiter, void* stream) {
+  if (bad_lim(limiter)) return fail(FAMRDP_E_KEY, "unknown limiter code " + std::to_string(limiter));
+  if (n < 0 || (n > 0 && (!a || !b || !out))) return fail(FAMRDP_E_INVALID, "limiter: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_limiter<<<nblk(n), 256, 0, S(stream)>>>(a, b, out, n, limiter);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_muscl2(const double* pv, double* pv_L, double* pv_R, int32_t nvar, int32_t n0, int32_t n1, int32_t n2,
+                  int32_t axis, int32_t limiter, void* stream) {
+  if (bad_lim(limiter)) return fail(FAMRDP_E_KEY, "unknown limiter code " + std::to_string(limiter));
+  if (axis < 1 || axis > 3 || nvar < 1 || n0 < 1 || n1 < 1 || n2 < 1)
+    return fail(FAMRDP_E_INVALID, "muscl2: bad shape or axis");
+  const int na = axis == 1 ? n0 : (axis == 2 ? n1 : n2);
+  if (na < 4) return fail(FAMRDP_E_INVALID, "muscl2: need at least 4 cells along the axis");
+  const int64_t nf = (int64_t)(axis == 1 ? n0 - 3 : n0) * (axis == 2 ? n1 - 3 : n1) * (axis == 3 ? n2 - 3 : n2);
+  k_muscl2<<<nblk(nf), 256, 0, S(stream)>>>(pv, pv_L, pv_R, nvar, n0, n1, n2, axis, limiter);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_roe_flux(const double* UL, const double* UR, const double* nxyz, double* F, int64_t n, double gamma,
+                    double eps_entropy, void* stream) {
+  if (n < 0 || (n > 0 && (!UL || !UR || !nxyz || !F))) return fail(FAMRDP_E_INVALID, "roe_flux: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_roe_flux<<<nblk(n), 256, 0, S(stream)>>>(UL, UR, nxyz, F, n, gamma, eps_entropy);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_invscd_rhs(const double* q, const double* kxyz, const double* vol, double* rhs, int32_t ni, int32_t nj,
+                      int32_t nk, double gamma, int32_t limiter, void* stream) {
+  if (bad_lim(limiter)) return fail(FAMRDP_E_KEY, "unknown limiter code " + std::to_string(limiter));
+  if (ni < 1 || nj < 1 || nk < 1 || !q || !kxyz || !vol || !rhs) return fail(FAMRDP_E_INVALID, "invscd_rhs: bad arguments");
+  const int64_t N = (int64_t)ni * nj * nk;
+  k_invscd_rhs<<<nblk(N), 256, 0, S(stream)>>>(q, kxyz, vol, rhs, ni, nj, nk, gamma, limiter);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_viscous_rhs(const double* q, const double* kxyz, double* rhs, int32_t ni, int32_t nj, int32_t nk,
+                       const famrdp_gas* gas, double* work, void* stream) {
+  if (ni < 1 || nj < 1 || nk < 1 || !q || !kxyz || !rhs || !gas || !work)
+    return fail(FAMRDP_E_INVALID, "viscous_rhs: bad arguments");
+  const int64_t N = (int64_t)ni * nj * nk;
+  k_viscous_flux<<<nblk(N), 256, 0, S(stream)>>>(q, kxyz, work, ni, nj, nk, to_gas(gas));
+  LAUNCH_CHECK();
+  k_viscous_div<<<nblk(N), 256, 0, S(stream)>>>(work, kxyz, rhs, ni, nj, nk);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_axpby(double* out, double a, const double* x, double b, const double* y, double c, const double* z,
+                 int64_t n, int32_t mode, void* stream) {
+  if (n < 0 || (n > 0 && (!out || !y || !z || (mode != 1 && !x)))) return fail(FAMRDP_E_INVALID, "axpby: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_axpby<<<nblk(n), 256, 0, S(stream)>>>(out, a, x, b, y, c, z, n, mode);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_compute_metrics(const double* xyz, double* jac, double* kxyz, double* vol, int32_t ni, int32_t nj,
+                           int32_t nk, void* stream) {
+  if (ni < 1 || nj < 1 || nk < 1 || !xyz || !jac || !kxyz || !vol) return fail(FAMRDP_E_INVALID, "compute_metrics: bad arguments");
+  const int64_t N = (int64_t)ni * nj * nk;
+  k_metrics<<<nblk(N), 256, 0, S(stream)>>>(xyz, jac, kxyz, vol, ni, nj, nk);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_cfl_dt(const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj, int32_t nk,
+                  int32_t ghost, const famrdp_gas* gas, double cfl, double* out, double* work, void* stream) {
+  if (!q || !kxyz || !vol || !gas || !out || !work || ghost < 0 || ni <= 2 * ghost || nj <= 2 * ghost ||
+      nk <= 2 * ghost)
+    return fail(FAMRDP_E_INVALID, "cfl_dt: bad arguments");
+  const int parts = 1024;
+  k_cfl_partial<<<parts, 256, 0, S(stream)>>>(q, kxyz, vol, ni, nj, nk, ghost, to_gas(gas), work, work + parts);
+  LAUNCH_CHECK();
+  k_cfl_final<<<1, 256, 0, S(stream)>>>(work, work + parts, parts, cfl, out);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_plan_timing(famrdp_plan* p, int32_t enable, double* stage_ms, int64_t* stage_count) {
+  if (!p) return fail(FAMRDP_E_INVALID, "plan is NULL");
+  if (enable) {
+    p->timing = true;
+    p->ev_used = 0;
+    return FAMRDP_OK;
+  }
+  double total = 0.0;
+  int64_t n = 0;
+  for (size_t i = 0; i + 1 < p->ev_used; i += 2) {
+    HIP_TRY(hipEventSynchronize(p->ev_pool[i + 1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p->ev_pool[i], p->ev_pool[i + 1]));
+    total += ms;
+    ++n;
+  }
+  p->timing = false;
+  p->ev_used = 0;
+  if (stage_ms) *stage_ms = total;
+  if (stage_count) *stage_count = n;
+  return FAMRDP_OK;
+}
+
+int famrdp_plan_destroy(famrdp_plan* p) {
+  if (!p) return FAMRDP_OK;
+  for (auto e : p->ev_pool) (void)hipEventDestroy(e);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  for (auto* x : p->q1) (void)hipFree(x);
+  for (auto* x : p->q2) (void)hipFree(x);
+  for (auto* x : p->G) (void)hipFree(x);
+  for (int f = 0; f < 6; ++f) if (p->d_bc[f]) (void)hipFree(p->d_bc[f]);
+  if (p->d_halo) (void)hipFree(p->d_halo);
+  if (p->d_halo_stage) (void)hipFree(p->d_halo_stage);
+  if (p->d_tiles) (void)hipFree(p->d_tiles);
+  if (p->d_blocks) (void)hipFree(p->d_blocks);
+  if (p->h_blocks_pinned) (void)hipHostFree(p->h_blocks_pinned);
+  delete p;
+  return FAMRDP_OK;
+}
+
+int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, int32_t ncut, const famrdp_cut* cuts,
+                       int32_t nbc, const famrdp_bc* bcs, const famrdp_gas* gas, int32_t limiter, int32_t ghost,
+                       int32_t plan_flags) {
+  if (!out) return fail(FAMRDP_E_INVALID, "plan_create: out is NULL");
+  *out = nullptr;
+  if (nblocks < 1 || !dims || !gas || ncut < 0 || nbc < 0 || (ncut > 0 && !cuts) || (nbc > 0 && !bcs))
+    return fail(FAMRDP_E_INVALID, "plan_create: bad arguments");
+  if (bad_lim(limiter)) return fail(FAMRDP_E_KEY, "unknown limiter code " + std::to_string(limiter));
+  if (ghost < 1) return fail(FAMRDP_E_INVALID, "ghost must be >= 1");
+  if (ghost != 2 && !(plan_flags & FAMRDP_PLAN_NO_WORKSPACE))
+    return fail(FAMRDP_E_UNSUPPORTED, "the step needs ghost=2 (the MUSCL-2 stencil width)");
+  famrdp_plan* p = new famrdp_plan();
+  p->nblocks = nblocks;
+  p->ghost = ghost;
+  p->limiter = limiter;
+  p->gas = to_gas(gas);
+  p->dims.assign(dims, dims + 3 * nblocks);
+  p->has_workspace = !(plan_flags & FAMRDP_PLAN_NO_WORKSPACE);
+  int rc = FAMRDP_OK;
+  auto cleanup = [&](int code) { famrdp_plan_destroy(p); return code; };
+  for (int b = 0; b < nblocks; ++b) {
+    const int32_t* d = &dims[3 * b];
+    if (d[0] < 5 || d[1] < 5 || d[2] < 5)
+      return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": each padded size must be >= 5"));
+    p->N.push_back((int64_t)d[0] * d[1] * d[2]);
+  }
+  // workspace and block table
+  p->h_blocks.resize(nblocks);
+  for (int b = 0; b < nblocks; ++b) {
+    double *q1 = nullptr, *q2 = nullptr, *G = nullptr;
+    if (p->has_workspace && (hipMalloc(&q1, sizeof(double) * 5 * p->N[b]) != hipSuccess ||
+        hipMalloc(&q2, sizeof(double) * 5 * p->N[b]) != hipSuccess ||
+        hipMalloc(&G, sizeof(double) * 9 * p->N[b]) != hipSuccess)) {
+      if (q1) (void)hipFree(q1);
+      if (q2) (void)hipFree(q2);
+      if (G) (void)hipFree(G);
+      return cleanup(fail(FAMRDP_E_HIP, "plan_create: workspace allocation failed"));
+    }
+    p->q1.push_back(q1);
+    p->q2.push_back(q2);
+    p->G.push_back(G);
+    BlockDesc& bd = p->h_blocks[b];
+    bd = BlockDesc{};
+    bd.q1 = q1; bd.q2 = q2; bd.G = G;
+    bd.ni = dims[3 * b]; bd.nj = dims[3 * b + 1]; bd.nk = dims[3 * b + 2];
+    bd.N = p->N[b];
+  }
+  if (hipMalloc(&p->d_blocks, sizeof(BlockDesc) * nblocks) != hipSuccess ||
+      hipHostMalloc(&p->h_blocks_pinned, sizeof(BlockDesc) * nblocks) != hipSuccess)
+    return cleanup(fail(FAMRDP_E_HIP, "plan_create: block table allocation failed"));
+  // halo gather table
+  std::vector<HaloEntry> halo;
+  rc = build_halo_table(p, ncut, cuts, halo);
+  if (rc) return cleanup(rc);
+  p->nhalo = (int)halo.size();
+  if (p->nhalo > 0) {
+    if (hipMalloc(&p->d_halo, sizeof(HaloEntry) * halo.size()) != hipSuccess ||
+        hipMalloc(&p->d_halo_stage, sizeof(double) * 5 * halo.size()) != hipSuccess)
+      return cleanup(fail(FAMRDP_E_HIP, "plan_create: halo table allocation failed"));
+    if (hipMemcpy(p->d_halo, halo.data(), sizeof(HaloEntry) * halo.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(FAMRDP_E_HIP, "plan_create: halo table upload failed"));
+  }
+  // BC tables grouped by face (Face order, dispatch.py:15)
+  std::vector<BcEntry> byface[6];
+  for (int i = 0; i < nbc; ++i) {
+    const famrdp_bc& c = bcs[i];
+    if (c.block < 0 || c.block >= nblocks || c.face < 0 || c.face > 5 || c.kind < 0 || c.kind > 4)
+      return cleanup(fail(FAMRDP_E_INVALID, "bc " + std::to_string(i) + ": block, face or kind out of range"));
+    BcEntry e{};
+    e.block = c.block; e.face = c.face; e.kind = c.kind; e.T_wall = c.T_wall;
+    for (int v = 0; v < 5; ++v) e.q_inf[v] = c.q_inf[v];
+    byface[c.face].push_back(e);
+    const int32_t* d = &dims[3 * c.block];
+    const int ax = c.face / 2;
+    const int64_t cells = (int64_t)d[(ax + 1) % 3] * d[(ax + 2) % 3];
+    p->bc_maxcells[c.face] = std::max(p->bc_maxcells[c.face], cells);
+  }
+  for (int f = 0; f < 6; ++f) {
+    // a block with two BCs on one face keeps the last (dict semantics)
+    std::vector<BcEntry> uniq;
+    for (auto& e : byface[f]) {
+      bool replaced = false;
+      for (auto& u : uniq) if (u.block == e.block) { u = e; replaced = true; }
+      if (!replaced) uniq.push_back(e);
+    }
+    p->nbc[f] = (int)uniq.size();
+    if (uniq.empty()) continue;
+    if (hipMalloc(&p->d_bc[f], sizeof(BcEntry) * uniq.size()) != hipSuccess ||
+        hipMemcpy(p->d_bc[f], uniq.data(), sizeof(BcEntry) * uniq.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(FAMRDP_E_HIP, "plan_create: bc table upload failed"));
+  }
+  // tile table for the batched launches
+  std::vector<Tile> tiles;
+  for (int b = 0; b < nblocks; ++b)
+    for (int64_t s = 0; s < p->N[b]; s += 256) tiles.push_back(Tile{b, (int32_t)s});
+  p->ntiles = (int)tiles.size();
+  if (hipMalloc(&p->d_tiles, sizeof(Tile) * tiles.size()) != hipSuccess ||
+      hipMemcpy(p->d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(FAMRDP_E_HIP, "plan_create: tile table upload failed"));
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+  }
+  p->persistent_grid = std::min(p->ntiles, ncu * 8);
+  *out = p;
+  return FAMRDP_OK;
+}
+
+int famrdp_halo_exchange(famrdp_plan* p, double* const* q, void* stream) {
+  if (!p) return fail(FAMRDP_E_INVALID, "plan is NULL");
+  int rc = bind(p, q, nullptr, nullptr, S(stream));
+  if (rc) return rc;
+  return launch_halo(p, S(stream));
+}
+
+int famrdp_apply_bc(famrdp_plan* p, double* const* q, void* stream) {
+  if (!p) return fail(FAMRDP_E_INVALID, "plan is NULL");
+  int rc = bind(p, q, nullptr, nullptr, S(stream));
+  if (rc) return rc;
+  return launch_bc(p, S(stream));
+}
+
+int famrdp_step(famrdp_plan* p, double* const* q, const double* const* kxyz, const double* const* vol, double dt,
+                int32_t flags, void* stream) {
+  if (!p || !kxyz || !vol) return fail(FAMRDP_E_INVALID, "step: NULL plan or metric pointer arrays");
+  if (!p->has_workspace) return fail(FAMRDP_E_INVALID, "step: plan was created without stage workspace");
+  for (int b = 0; b < p->nblocks; ++b)
+    if (!kxyz[b] || !vol[b]) return fail(FAMRDP_E_INVALID, "step: NULL kxyz/vol for block " + std::to_string(b));
+  int rc = bind(p, q, kxyz, vol, S(stream));
+  if (rc) return rc;
+  return run_step(p, dt, flags, S(stream));
+}
+
+int famrdp_step_graph(famrdp_plan* p, double* const* q, const double* const* kxyz, const double* const* vol,
+                      double dt, int32_t flags, int32_t nsteps, void* stream) {
+  if (!p || !kxyz || !vol || nsteps < 0) return fail(FAMRDP_E_INVALID, "step_graph: bad arguments");
+  if (!p->has_workspace) return fail(FAMRDP_E_INVALID, "step_graph: plan was created without stage workspace");
+  for (int b = 0; b < p->nblocks; ++b)
+    if (!kxyz[b] || !vol[b]) return fail(FAMRDP_E_INVALID, "step_graph: NULL kxyz/vol for block " + std::to_string(b));
+  int rc = bind(p, q, kxyz, vol, S(stream));
+  if (rc) return rc;
+  std::vector<const void*> key;
+  for (int b = 0; b < p->nblocks; ++b) {
+    key.push_back(q[b]);
+    key.push_back(kxyz[b]);
+    key.push_back(vol[b]);
+  }
+  if (!p->graph_exec || key != p->graph_key || dt != p->graph_dt || flags != p->graph_flags) {
+    if (p->graph_exec) { (void)hipGraphExecDestroy(p->graph_exec); p->graph_exec = nullptr; }
+    if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+    if (!p->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+    p->capturing = true;
+    rc = run_step(p, dt, flags, p->cap_stream);
+    p->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(FAMRDP_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    p->graph = g;
+    HIP_TRY(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
+    p->graph_key = key;
+    p->graph_dt = dt;
+    p->graph_flags = flags;
+  }
+  for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(p->graph_exec, S(stream)));
+  return FAMRDP_OK;
+}
+
+}  // extern "C"

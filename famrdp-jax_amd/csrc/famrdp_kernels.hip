@@ -1,0 +1,514 @@
+// FP64 kernels of famrdp_jax's RHS + TVD-RK3 path for gfx950.
+//
+// Kernel families:
+//   * unit kernels behind the reference's pointwise API (eos, MUSCL-2, Roe);
+//   * the direct stage pair (viscous flux tensor, then inviscid + viscous
+//     divergence + RK3 combination), one thread per padded cell, in a
+//     per-block form and a batched form driven by a tile table;
+//   * halo gather/scatter (pre-exchange semantics) and ghost-cell BC faces;
+//   * grid metrics and the CFL / positivity reduction.
+#include "famrdp_kernels.h"
+
+namespace famrdp {
+
+__device__ __forceinline__ int wrap_m(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__device__ __forceinline__ int wrap_p(int i, int n) { return i == n - 1 ? 0 : i + 1; }
+
+__device__ __forceinline__ void load_prim(const double* __restrict__ q, int64_t N, int64_t c, double g,
+                                          double pv[5]) {
+  double qc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qc[v] = q[v * N + c];
+  cons_to_prim(qc, g, pv);
+}
+
+// ---------------------------------------------------------------------------
+// Unit kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_cons_to_prim(const double* __restrict__ q, double* __restrict__ pv, int64_t n, double g) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double a[5], b[5];
+  for (int v = 0; v < 5; ++v) a[v] = q[v * n + t];
+  cons_to_prim(a, g, b);
+  for (int v = 0; v < 5; ++v) pv[v * n + t] = b[v];
+}
+
+__global__ void k_prim_to_cons(const double* __restrict__ pv, double* __restrict__ q, int64_t n, double g) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double a[5], b[5];
+  for (int v = 0; v < 5; ++v) a[v] = pv[v * n + t];
+  prim_to_cons(a, g, b);
+  for (int v = 0; v < 5; ++v) q[v * n + t] = b[v];
+}
+
+__global__ void k_sutherland(const double* __restrict__ T, double* __restrict__ mu, int64_t n, double mu0,
+                             double T0, double Ts) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) mu[t] = sutherland(T[t], mu0, T0, Ts);
+}
+
+__global__ void k_speed_of_sound(const double* __restrict__ rho, const double* __restrict__ p,
+                                 double* __restrict__ c, int64_t n, double g) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) c[t] = sqrt(g * p[t] / rho[t]);
+}
+
+__global__ void k_limiter(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out,
+                          int64_t n, int lim) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) out[t] = limit(lim, a[t], b[t]);
+}
+
+// pv (nvar, n0, n1, n2); faces along `axis` (1..3) -> (nvar, ..., n_axis-3, ...)
+__global__ void k_muscl2(const double* __restrict__ pv, double* __restrict__ pL, double* __restrict__ pR,
+                         int nvar, int n0, int n1, int n2, int axis, int lim) {
+  const int f0 = axis == 1 ? n0 - 3 : n0, f1 = axis == 2 ? n1 - 3 : n1, f2 = axis == 3 ? n2 - 3 : n2;
+  const int64_t nf = (int64_t)f0 * f1 * f2;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nf) return;
+  const int a2 = (int)(t % f2), a1 = (int)((t / f2) % f1), a0 = (int)(t / ((int64_t)f2 * f1));
+  const int64_t s0 = (int64_t)n1 * n2, s1 = n2;
+  const int64_t stride = axis == 1 ? s0 : (axis == 2 ? s1 : 1);
+  // face f along the axis sits between cells f+1 and f+2: base cell index = f
+  const int64_t base = (int64_t)a0 * s0 + (int64_t)a1 * s1 + a2;
+  const int64_t N = (int64_t)n0 * n1 * n2;
+  for (int v = 0; v < nvar; ++v) {
+    const double* p = pv + v * N + base;
+    const double pmm = p[0], pc = p[stride], pp1 = p[2 * stride], pp2 = p[3 * stride];
+    const double sL = limit(lim, pc - pmm, pp1 - pc);
+    const double sR = limit(lim, pp1 - pc, pp2 - pp1);
+    pL[v * nf + t] = pc + 0.5 * sL;
+    pR[v * nf + t] = pp1 - 0.5 * sR;
+  }
+}
+
+__global__ void k_roe_flux(const double* __restrict__ UL, const double* __restrict__ UR,
+                           const double* __restrict__ nx, double* __restrict__ F, int64_t n, double g,
+                           double eps_entropy) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double a[5], b[5], nn[3], f[5];
+  for (int v = 0; v < 5; ++v) { a[v] = UL[v * n + t]; b[v] = UR[v * n + t]; }
+  for (int v = 0; v < 3; ++v) nn[v] = nx[v * n + t];
+  roe_flux(a, b, nn, g, eps_entropy, f);
+  for (int v = 0; v < 5; ++v) F[v * n + t] = f[v];
+}
+
+// ---------------------------------------------------------------------------
+// Direct stage: per-cell device functions
+// ---------------------------------------------------------------------------
+
+// Viscous flux tensor G at cell c (rhs/viscous.py:30-108, roll wrap).
+__device__ __forceinline__ void cell_viscous_flux(const double* __restrict__ q, const double* __restrict__ kxyz,
+                                                  double* __restrict__ G, int ni, int nj, int nk, int64_t N,
+                                                  int64_t c, const Gas& gas) {
+  const int k = (int)(c % nk), j = (int)((c / nk) % nj), i = (int)(c / ((int64_t)nk * nj));
+  const int64_t sj = nk, si = (int64_t)nj * nk;
+  const int64_t cm[3] = {c + (wrap_m(i, ni) - i) * si, c + (wrap_m(j, nj) - j) * sj, c + (wrap_m(k, nk) - k)};
+  const int64_t cp[3] = {c + (wrap_p(i, ni) - i) * si, c + (wrap_p(j, nj) - j) * sj, c + (wrap_p(k, nk) - k)};
+  double pv[5];
+  load_prim(q, N, c, gas.gamma, pv);
+  const double T = temperature(pv, gas.R);
+  double d[3][4];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    double pp[5], pm[5];
+    load_prim(q, N, cp[l], gas.gamma, pp);
+    load_prim(q, N, cm[l], gas.gamma, pm);
+    d[l][0] = 0.5 * (pp[1] - pm[1]);
+    d[l][1] = 0.5 * (pp[2] - pm[2]);
+    d[l][2] = 0.5 * (pp[3] - pm[3]);
+    d[l][3] = 0.5 * (temperature(pp, gas.R) - temperature(pm, gas.R));
+  }
+  double kk[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kk[m] = kxyz[m * N + c];
+  double g9[9];
+  viscous_flux(pv, T, d, kk, gas, g9);
+#pragma unroll
+  for (int m = 0; m < 9; ++m) G[m * N + c] = g9[m];
+}
+
+// Inviscid RHS at cell c (rhs/invscd.py:53-131): axis contributions added in
+// axis order, only where 2 <= c_d <= n_d - 3.
+__device__ __forceinline__ void cell_invscd(const double* __restrict__ q, const double* __restrict__ kxyz,
+                                            const double* __restrict__ vol, int ni, int nj, int nk, int64_t N,
+                                            int64_t c, double g, int lim, double inv[5]) {
+  const int k = (int)(c % nk), j = (int)((c / nk) % nj), i = (int)(c / ((int64_t)nk * nj));
+  const int idx[3] = {i, j, k};
+  const int n[3] = {ni, nj, nk};
+  const int64_t st[3] = {(int64_t)nj * nk, nk, 1};
+#pragma unroll
+  for (int v = 0; v < 5; ++v) inv[v] = 0.0;
+  const double rvol = vol[c] + 1e-300;
+  for (int d = 0; d < 3; ++d) {
+    if (idx[d] < 2 || idx[d] > n[d] - 3) continue;
+    double pv[5][5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) load_prim(q, N, c + (s - 2) * st[d], g, pv[s]);
+    double kd[3][3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) kd[s][m] = kxyz[(d * 3 + m) * N + c + (s - 1) * st[d]];
+    double Fm[5], Fp[5];
+    face_flux(pv[0], pv[1], pv[2], pv[3], kd[0], kd[1], g, lim, Fm);   // face c-1/2
+    face_flux(pv[1], pv[2], pv[3], pv[4], kd[1], kd[2], g, lim, Fp);   // face c+1/2
+#pragma unroll
+    for (int v = 0; v < 5; ++v) inv[v] = inv[v] + (-(Fp[v] - Fm[v]) / rvol);
+  }
+}
+
+// Viscous divergence at cell c from G (rhs/viscous.py:86-108): for each row,
+// sum over j of sum_l kxyz[l, j] * 0.5 (G_j(+e_l) - G_j(-e_l)).
+__device__ __forceinline__ void cell_viscous_div(const double* __restrict__ G, const double* __restrict__ kxyz,
+                                                 int ni, int nj, int nk, int64_t N, int64_t c, double vis[4]) {
+  const int k = (int)(c % nk), j = (int)((c / nk) % nj), i = (int)(c / ((int64_t)nk * nj));
+  const int64_t sj = nk, si = (int64_t)nj * nk;
+  const int64_t cm[3] = {c + (wrap_m(i, ni) - i) * si, c + (wrap_m(j, nj) - j) * sj, c + (wrap_m(k, nk) - k)};
+  const int64_t cp[3] = {c + (wrap_p(i, ni) - i) * si, c + (wrap_p(j, nj) - j) * sj, c + (wrap_p(k, nk) - k)};
+  double kk[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kk[m] = kxyz[m * N + c];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    double res = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      const double* Gc = G + gidx(r, jj) * N;
+      const double d0 = 0.5 * (Gc[cp[0]] - Gc[cm[0]]);
+      const double d1 = 0.5 * (Gc[cp[1]] - Gc[cm[1]]);
+      const double d2 = 0.5 * (Gc[cp[2]] - Gc[cm[2]]);
+      res = res + (d0 * kk[0 * 3 + jj] + d1 * kk[1 * 3 + jj] + d2 * kk[2 * 3 + jj]);
+    }
+    vis[r] = res;
+  }
+}
+
+// Shu-Osher combination (solver/step.py:45-53).
+__device__ __forceinline__ double rk_combine(int stage, double q0, double qs, double dt, double r) {
+  if (stage == 1) return qs + dt * r;
+  if (stage == 2) return 0.75 * q0 + 0.25 * (qs + dt * r);
+  return (1.0 / 3.0) * q0 + (2.0 / 3.0) * (qs + dt * r);
+}
+
+// q0 may alias qout (stage 3 writes the state in place): both are read for
+// cell c only, and every read happens before the first write.
+__device__ __forceinline__ void cell_stage(const double* __restrict__ qin, const double* q0, double* qout,
+                                           const double* __restrict__ kxyz, const double* __restrict__ vol,
+                                           const double* __restrict__ G, int ni, int nj, int nk, int64_t N, int64_t c,
+                                           double g, int lim, double dt, int stage) {
+  double inv[5], vis[4];
+  cell_invscd(qin, kxyz, vol, ni, nj, nk, N, c, g, lim, inv);
+  cell_viscous_div(G, kxyz, ni, nj, nk, N, c, vis);
+  double rhs[5];
+  rhs[0] = inv[0] + 0.0;
+  rhs[1] = inv[1] + vis[0];
+  rhs[2] = inv[2] + vis[1];
+  rhs[3] = inv[3] + vis[2];
+  rhs[4] = inv[4] + vis[3];
+  double qs[5], qq0[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    qs[v] = qin[v * N + c];
+    qq0[v] = stage == 1 ? 0.0 : q0[v * N + c];
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qout[v * N + c] = rk_combine(stage, qq0[v], qs[v], dt, rhs[v]);
+}
+
+// ---------------------------------------------------------------------------
+// Direct stage kernels: per-block launches
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_viscous_flux(const double* __restrict__ q, const double* __restrict__ kxyz,
+                                                      double* __restrict__ G, int ni, int nj, int nk, Gas gas) {
+  const int64_t N = (int64_t)ni * nj * nk;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c < N) cell_viscous_flux(q, kxyz, G, ni, nj, nk, N, c, gas);
+}
+
+__global__ void __launch_bounds__(256) k_stage(const double* __restrict__ qin, const double* q0,
+                                               double* qout, const double* __restrict__ kxyz,
+                                               const double* __restrict__ vol, const double* __restrict__ G, int ni,
+                                               int nj, int nk, double g, int lim, double dt, int stage) {
+  const int64_t N = (int64_t)ni * nj * nk;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c < N) cell_stage(qin, q0, qout, kxyz, vol, G, ni, nj, nk, N, c, g, lim, dt, stage);
+}
+
+// RHS only (the unit entry points compute_invscd_rhs / compute_viscous_rhs)
+__global__ void __launch_bounds__(256) k_invscd_rhs(const double* __restrict__ q, const double* __restrict__ kxyz,
+                                                    const double* __restrict__ vol, double* __restrict__ rhs, int ni,
+                                                    int nj, int nk, double g, int lim) {
+  const int64_t N = (int64_t)ni * nj * nk;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double inv[5];
+  cell_invscd(q, kxyz, vol, ni, nj, nk, N, c, g, lim, inv);
+  for (int v = 0; v < 5; ++v) rhs[v * N + c] = inv[v];
+}
+
+__global__ void __launch_bounds__(256) k_viscous_div(const double* __restrict__ G, const double* __restrict__ kxyz,
+                                                     double* __restrict__ rhs, int ni, int nj, int nk) {
+  const int64_t N = (int64_t)ni * nj * nk;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double vis[4];
+  cell_viscous_div(G, kxyz, ni, nj, nk, N, c, vis);
+  rhs[c] = 0.0;
+  for (int v = 0; v < 4; ++v) rhs[(v + 1) * N + c] = vis[v];
+}
+
+// Generic RK combination for rk3_step(q, L, dt): out = a*x + b*(y + c*z), or
+// y + c*z when mode == 1 (time/rk.py:17-19).
+__global__ void k_axpby(double* __restrict__ out, double a, const double* __restrict__ x, double b,
+                        const double* __restrict__ y, double c, const double* __restrict__ z, int64_t n, int mode) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = mode == 1 ? y[t] + c * z[t] : a * x[t] + b * (y[t] + c * z[t]);
+}
+
+// ---------------------------------------------------------------------------
+// Batched direct stage kernels: persistent grid over a tile table that spans
+// every block of the plan (the block-offset table is `blocks`).
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_viscous_flux_batched(const BlockDesc* __restrict__ blocks,
+                                                              const Tile* __restrict__ tiles, int ntiles, int stage,
+                                                              Gas gas) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const BlockDesc b = blocks[tl.block];
+    const int64_t c = (int64_t)tl.start + threadIdx.x;
+    if (c < b.N) {
+      const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
+      cell_viscous_flux(qin, b.kxyz, b.G, b.ni, b.nj, b.nk, b.N, c, gas);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_stage_batched(const BlockDesc* __restrict__ blocks,
+                                                       const Tile* __restrict__ tiles, int ntiles, double g, int lim,
+                                                       double dt, int stage) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const BlockDesc b = blocks[tl.block];
+    const int64_t c = (int64_t)tl.start + threadIdx.x;
+    if (c < b.N) {
+      const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
+      double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
+      cell_stage(qin, b.q, qout, b.kxyz, b.vol, b.G, b.ni, b.nj, b.nk, b.N, c, g, lim, dt, stage);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Halo exchange (mesh/halo.py:56-92): gather every final ghost value from the
+// pre-exchange state into `stage`, then scatter; two launches give the
+// reference's "all copies read the original arrays" semantics.
+// ---------------------------------------------------------------------------
+
+__global__ void k_halo_gather(const BlockDesc* __restrict__ blocks, const HaloEntry* __restrict__ e, int n,
+                              double* __restrict__ stage) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const HaloEntry h = e[t];
+  const BlockDesc s = blocks[h.sb];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) stage[(int64_t)v * n + t] = s.q[v * s.N + h.sc];
+}
+
+__global__ void k_halo_scatter(const BlockDesc* __restrict__ blocks, const HaloEntry* __restrict__ e, int n,
+                               const double* __restrict__ stage) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const HaloEntry h = e[t];
+  const BlockDesc b = blocks[h.tb];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) b.q[v * b.N + h.tc] = stage[(int64_t)v * n + t];
+}
+
+// ---------------------------------------------------------------------------
+// Ghost-cell BCs (physics/bc/*.py).  One launch per face index (Face order),
+// covering that face of every block that has a BC there; blockIdx.y = entry.
+// ---------------------------------------------------------------------------
+
+__global__ void k_bc_face(const BlockDesc* __restrict__ blocks, const BcEntry* __restrict__ entries, int ghost,
+                          double gamma, double R) {
+  const BcEntry e = entries[blockIdx.y];
+  const BlockDesc b = blocks[e.block];
+  const int ax = e.face / 2;            // 0: i, 1: j, 2: k
+  const bool lo = (e.face % 2) == 0;
+  const int n[3] = {b.ni, b.nj, b.nk};
+  const int64_t st[3] = {(int64_t)b.nj * b.nk, b.nk, 1};
+  const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+  const int64_t ncell = (int64_t)n[t1] * n[t2];
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  const int a = (int)(t / n[t2]), c2 = (int)(t % n[t2]);
+  const int64_t base = a * st[t1] + c2 * st[t2];
+  const int na = n[ax];
+  for (int gl = 0; gl < ghost; ++gl) {
+    // bc/base.py:21-33 mirror pairs; extrapolation reads the first interior layer
+    const int outer = lo ? gl : na - 1 - gl;
+    int inner = lo ? 2 * ghost - 1 - gl : na - 2 * ghost + gl;
+    if (e.kind == BC_EXTRAPOLATE) inner = lo ? ghost : na - 1 - ghost;
+    const int64_t co = base + outer * st[ax], ci = base + inner * st[ax];
+    double qi[5], qo[5];
+    for (int v = 0; v < 5; ++v) qi[v] = b.q[v * b.N + ci];
+    switch (e.kind) {
+      case BC_WALL_ADIABATIC:
+        qo[0] = qi[0]; qo[1] = -qi[1]; qo[2] = -qi[2]; qo[3] = -qi[3]; qo[4] = qi[4];
+        break;
+      case BC_WALL_ISOTHERMAL: {   // bc/wall.py:35-58
+        const double rho_i = qi[0];
+        const double ke_i = 0.5 * (qi[1] * qi[1] + qi[2] * qi[2] + qi[3] * qi[3]) / rho_i;
+        const double p_g = (gamma - 1.0) * (qi[4] - ke_i);
+        const double rho_g = p_g / (R * e.T_wall);
+        const double scale = rho_g / rho_i;
+        const double mu = -qi[1] * scale, mv = -qi[2] * scale, mw = -qi[3] * scale;
+        const double ke_g = 0.5 * (mu * mu + mv * mv + mw * mw) / rho_g;
+        qo[0] = rho_g; qo[1] = mu; qo[2] = mv; qo[3] = mw; qo[4] = p_g / (gamma - 1.0) + ke_g;
+        break;
+      }
+      case BC_MIRROR_NORMAL:
+        for (int v = 0; v < 5; ++v) qo[v] = (v == ax + 1) ? qi[v] * -1.0 : qi[v] * 1.0;
+        break;
+      case BC_FARFIELD:
+        for (int v = 0; v < 5; ++v) qo[v] = e.q_inf[v] * 1.0;
+        break;
+      default:  // BC_EXTRAPOLATE
+        for (int v = 0; v < 5; ++v) qo[v] = qi[v];
+        break;
+    }
+    for (int v = 0; v < 5; ++v) b.q[v * b.N + co] = qo[v];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Grid metrics (mesh/metric.py:18-70): 4th-order central differences with
+// roll wrap, J[m][l] = dx_m/dxi_l, jac = det J, kxyz = inv(J) as [l][m].
+// ---------------------------------------------------------------------------
+
+__global__ void k_metrics(const double* __restrict__ xyz, double* __restrict__ jac, double* __restrict__ kxyz,
+                          double* __restrict__ vol, int ni, int nj, int nk) {
+  const int64_t N = (int64_t)ni * nj * nk;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int k = (int)(c % nk), j = (int)((c / nk) % nj), i = (int)(c / ((int64_t)nk * nj));
+  const int idx[3] = {i, j, k};
+  const int n[3] = {ni, nj, nk};
+  const int64_t st[3] = {(int64_t)nj * nk, nk, 1};
+  double J[3][3];
+  for (int l = 0; l < 3; ++l) {
+    int64_t o[4];
+    const int sh[4] = {2, 1, -1, -2};
+    for (int s = 0; s < 4; ++s) {
+      int ii = idx[l] + sh[s];
+      ii = ((ii % n[l]) + n[l]) % n[l];
+      o[s] = c + (int64_t)(ii - idx[l]) * st[l];
+    }
+    for (int m = 0; m < 3; ++m) {
+      const double* f = xyz + m * N;
+      J[m][l] = (-f[o[0]] + 8.0 * f[o[1]] - 8.0 * f[o[2]] + f[o[3]]) / 12.0;
+    }
+  }
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double inv[3][3] = {
+      {c00 / det, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det, (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det},
+      {c01 / det, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det},
+      {c02 / det, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det, (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det}};
+  jac[c] = det;
+  vol[c] = fabs(det);
+  for (int l = 0; l < 3; ++l)
+    for (int m = 0; m < 3; ++m) kxyz[(l * 3 + m) * N + c] = inv[l][m];
+}
+
+// ---------------------------------------------------------------------------
+// CFL time step and positivity monitor over interior cells (two passes, no
+// atomics): per-workgroup max spectral radius and bad-cell count, then one
+// workgroup reduces them.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_cfl_partial(const double* __restrict__ q, const double* __restrict__ kxyz,
+                                                     const double* __restrict__ vol, int ni, int nj, int nk, int ghost,
+                                                     Gas gas, double* __restrict__ part_lam,
+                                                     double* __restrict__ part_bad) {
+  __shared__ double slam[256];
+  __shared__ double sbad[256];
+  const int64_t N = (int64_t)ni * nj * nk;
+  const int ii = ni - 2 * ghost, jj = nj - 2 * ghost, kk = nk - 2 * ghost;
+  const int64_t nint = (int64_t)ii * jj * kk;
+  double lam = 0.0, bad = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nint; t += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(t % kk) + ghost, j = (int)((t / kk) % jj) + ghost, i = (int)(t / ((int64_t)kk * jj)) + ghost;
+    const int64_t c = ((int64_t)i * nj + j) * nk + k;
+    double pv[5];
+    load_prim(q, N, c, gas.gamma, pv);
+    const bool ok = isfinite(pv[0]) && isfinite(pv[1]) && isfinite(pv[2]) && isfinite(pv[3]) && isfinite(pv[4]) &&
+                    pv[0] > 0.0 && pv[4] > 0.0;
+    if (!ok) { bad += 1.0; continue; }
+    const double cs = sqrt(gas.gamma * pv[4] / pv[0]);
+    const double T = pv[4] / (pv[0] * gas.R);
+    const double mu = sutherland(T, gas.mu0, gas.T0, gas.Ts);
+    double inv = 0.0, ksq = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double k0 = kxyz[(d * 3 + 0) * N + c], k1 = kxyz[(d * 3 + 1) * N + c], k2 = kxyz[(d * 3 + 2) * N + c];
+      const double kn2 = k0 * k0 + k1 * k1 + k2 * k2;
+      inv = inv + (fabs(pv[1] * k0 + pv[2] * k1 + pv[3] * k2) + cs * sqrt(kn2));
+      ksq = ksq + kn2;
+    }
+    const double l = inv / vol[c] + fmax(4.0 / 3.0, gas.gamma / gas.Pr_lam) * mu / pv[0] * ksq;
+    lam = fmax(lam, l);
+  }
+  slam[threadIdx.x] = lam;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      slam[threadIdx.x] = fmax(slam[threadIdx.x], slam[threadIdx.x + s]);
+      sbad[threadIdx.x] += sbad[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_lam[blockIdx.x] = slam[0];
+    part_bad[blockIdx.x] = sbad[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cfl_final(const double* __restrict__ part_lam,
+                                                   const double* __restrict__ part_bad, int n, double cfl,
+                                                   double* __restrict__ out) {
+  __shared__ double slam[256];
+  __shared__ double sbad[256];
+  double lam = 0.0, bad = 0.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    lam = fmax(lam, part_lam[t]);
+    bad += part_bad[t];
+  }
+  slam[threadIdx.x] = lam;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      slam[threadIdx.x] = fmax(slam[threadIdx.x], slam[threadIdx.x + s]);
+      sbad[threadIdx.x] += sbad[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = cfl / slam[0];
+    out[1] = slam[0];
+    out[2] = sbad[0];
+  }
+}
+
+}  // namespace famrdp

@@ -1,0 +1,199 @@
+// Pointwise FP64 numerics of famrdp_jax's RHS path, shared by every kernel.
+//
+// Each function restates one reference formula in the reference's evaluation
+// order (so results track the CPU restatement to rounding), written as plain
+// scalar FP64 code for one cell or one face.  Citations are to the reference
+// repository (famrdp_jax/...).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define FD __host__ __device__ __forceinline__
+
+namespace famrdp {
+
+enum Limiter : int { LIM_MINMOD = 0, LIM_VANLEER = 1 };
+
+// physics/eos.py:12-17
+FD void cons_to_prim(const double q[5], double g, double pv[5]) {
+  const double rho = q[0];
+  const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
+  const double ke = 0.5 * (u * u + v * v + w * w);
+  pv[0] = rho; pv[1] = u; pv[2] = v; pv[3] = w;
+  pv[4] = (g - 1.0) * (q[4] - rho * ke);
+}
+
+// physics/eos.py:5-9
+FD void prim_to_cons(const double pv[5], double g, double U[5]) {
+  const double rho = pv[0], u = pv[1], v = pv[2], w = pv[3], p = pv[4];
+  const double ke = 0.5 * (u * u + v * v + w * w);
+  U[0] = rho; U[1] = rho * u; U[2] = rho * v; U[3] = rho * w;
+  U[4] = p / (g - 1.0) + rho * ke;
+}
+
+// physics/eos.py:24-25; (T/T0)**1.5 evaluated as x*sqrt(x)
+FD double sutherland(double T, double mu0, double T0, double Ts) {
+  const double x = T / T0;
+  return mu0 * (x * sqrt(x)) * (T0 + Ts) / (T + Ts);
+}
+
+FD double sgn(double a) { return a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : 0.0); }
+
+// physics/reconstruct/muscl2.py:5-7: s = sign(a)*max(sign(a)sign(b),0); s*min(|a|,|b|)
+FD double minmod(double a, double b) {
+  const double sa = sgn(a);
+  const double s = sa * fmax(sa * sgn(b), 0.0);
+  return s * fmin(fabs(a), fabs(b));
+}
+
+// physics/reconstruct/muscl2.py:10-12
+FD double vanleer(double a, double b) {
+  const double ab = a + b;
+  const double safe = fabs(ab) > 1e-30 ? ab : 1.0;
+  return (a * b > 0.0) ? (2.0 * a * b / safe) : 0.0;
+}
+
+FD double limit(int lim, double a, double b) {
+  return lim == LIM_VANLEER ? vanleer(a, b) : minmod(a, b);
+}
+
+// flux/roe.py:5-18
+FD void physical_flux(const double U[5], const double n[3], double g, double F[5]) {
+  const double rho = fmax(U[0], 1e-300);
+  const double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho;
+  const double ke = 0.5 * (u * u + v * v + w * w);
+  const double p = (g - 1.0) * (U[4] - rho * ke);
+  const double un = u * n[0] + v * n[1] + w * n[2];
+  const double H = (U[4] + p) / rho;
+  F[0] = rho * un;
+  F[1] = rho * u * un + p * n[0];
+  F[2] = rho * v * un + p * n[1];
+  F[3] = rho * w * un + p * n[2];
+  F[4] = rho * H * un;
+}
+
+// flux/roe.py:21-85: Roe flux for a unit normal n, Harten fix on lambda1/5.
+FD void roe_flux(const double UL[5], const double UR[5], const double n[3], double g,
+                 double eps_entropy, double F[5]) {
+  const double rhoL = fmax(UL[0], 1e-300), rhoR = fmax(UR[0], 1e-300);
+  const double uL = UL[1] / rhoL, vL = UL[2] / rhoL, wL = UL[3] / rhoL;
+  const double uR = UR[1] / rhoR, vR = UR[2] / rhoR, wR = UR[3] / rhoR;
+  const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+  const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+  const double pL = (g - 1.0) * (UL[4] - rhoL * keL);
+  const double pR = (g - 1.0) * (UR[4] - rhoR * keR);
+  const double HL = (UL[4] + pL) / rhoL, HR = (UR[4] + pR) / rhoR;
+
+  const double sqL = sqrt(rhoL), sqR = sqrt(rhoR);
+  const double ss = fmax(sqL + sqR, 1e-150);
+  const double uh = (sqL * uL + sqR * uR) / ss;
+  const double vh = (sqL * vL + sqR * vR) / ss;
+  const double wh = (sqL * wL + sqR * wR) / ss;
+  const double Hh = (sqL * HL + sqR * HR) / ss;
+  const double q2 = uh * uh + vh * vh + wh * wh;
+  const double c2 = fmax((g - 1.0) * (Hh - 0.5 * q2), 1e-30);
+  const double ch = sqrt(c2);
+  const double unh = uh * n[0] + vh * n[1] + wh * n[2];
+  const double rhoh = sqL * sqR;
+
+  const double eps = eps_entropy * (fabs(unh) + ch);
+  const double lam1 = unh - ch, lam5 = unh + ch;
+  const double al1 = fabs(lam1) >= eps ? fabs(lam1) : 0.5 * (lam1 * lam1 / eps + eps);
+  const double al5 = fabs(lam5) >= eps ? fabs(lam5) : 0.5 * (lam5 * lam5 / eps + eps);
+  const double al2 = fabs(unh);
+
+  double dU[5];
+  for (int v = 0; v < 5; ++v) dU[v] = UR[v] - UL[v];
+  const double dp = (g - 1.0) * (dU[4] - uh * dU[1] - vh * dU[2] - wh * dU[3] + 0.5 * q2 * dU[0]);
+  const double dun = (dU[1] * n[0] + dU[2] * n[1] + dU[3] * n[2]) - unh * dU[0];
+  const double a1 = (dp - rhoh * ch * dun) / (2.0 * c2);
+  const double a5 = (dp + rhoh * ch * dun) / (2.0 * c2);
+  const double a23 = dU[0] - dp / c2;
+
+  const double R1[5] = {1.0, uh - ch * n[0], vh - ch * n[1], wh - ch * n[2], Hh - ch * unh};
+  const double R5[5] = {1.0, uh + ch * n[0], vh + ch * n[1], wh + ch * n[2], Hh + ch * unh};
+  const double R2[5] = {1.0, uh, vh, wh, 0.5 * q2};
+
+  const double dut = dU[1] / rhoh - uh * dU[0] / rhoh;
+  const double dvt_ = dU[2] / rhoh - vh * dU[0] / rhoh;
+  const double dwt = dU[3] / rhoh - wh * dU[0] / rhoh;
+  const double dunf = dut * n[0] + dvt_ * n[1] + dwt * n[2];
+  const double tx = dut - dunf * n[0], ty = dvt_ - dunf * n[1], tz = dwt - dunf * n[2];
+  const double dvt[5] = {0.0, rhoh * tx, rhoh * ty, rhoh * tz, rhoh * (uh * tx + vh * ty + wh * tz)};
+
+  double FL[5], FR[5];
+  physical_flux(UL, n, g, FL);
+  physical_flux(UR, n, g, FR);
+  for (int v = 0; v < 5; ++v) {
+    const double absA = al1 * a1 * R1[v] + al5 * a5 * R5[v] + al2 * (a23 * R2[v] + dvt[v]);
+    F[v] = 0.5 * (FL[v] + FR[v]) - 0.5 * absA;
+  }
+}
+
+// One face of rhs/invscd.py:53-97 (the MUSCL-2 slopes of the two cells either
+// side, the averaged face metric and the area-weighted Roe flux).  pv0..pv3
+// are the primitives of cells a-1, a, a+1, a+2; k0/k1 are kxyz[d, :] at
+// cells a and a+1.  Returns F * |n| for the face between a and a+1.
+FD void face_flux(const double pv0[5], const double pv1[5], const double pv2[5], const double pv3[5],
+                  const double k0[3], const double k1[3], double g, int lim, double Fa[5]) {
+  double pL[5], pR[5];
+  for (int v = 0; v < 5; ++v) {
+    const double sL = limit(lim, pv1[v] - pv0[v], pv2[v] - pv1[v]);
+    const double sR = limit(lim, pv2[v] - pv1[v], pv3[v] - pv2[v]);
+    pL[v] = pv1[v] + 0.5 * sL;
+    pR[v] = pv2[v] - 0.5 * sR;
+  }
+  double UL[5], UR[5];
+  prim_to_cons(pL, g, UL);
+  prim_to_cons(pR, g, UR);
+  double nf[3] = {0.5 * (k0[0] + k1[0]), 0.5 * (k0[1] + k1[1]), 0.5 * (k0[2] + k1[2])};
+  const double nm = sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]) + 1e-300;
+  const double nu[3] = {nf[0] / nm, nf[1] / nm, nf[2] / nm};
+  double F[5];
+  roe_flux(UL, UR, nu, g, 0.1, F);
+  for (int v = 0; v < 5; ++v) Fa[v] = F[v] * nm;
+}
+
+struct Gas {
+  double gamma, R, mu0, T0, Ts, Pr_lam;
+};
+
+// rhs/viscous.py:30-108 at one cell: given the primitives of the cell, its
+// central differences d[l][s] = 0.5*(s(+e_l) - s(-e_l)) of s in (u,v,w,T) and
+// kxyz at the cell, produce the viscous flux tensor G = (tau00, tau01, tau02,
+// tau11, tau12, tau22, E0, E1, E2) with E_j = u_i tau_ij - heat_j.
+FD void viscous_flux(const double pv[5], double T, const double d[3][4], const double k[9],
+                     const Gas& gas, double G[9]) {
+  const double mu = sutherland(T, gas.mu0, gas.T0, gas.Ts);
+  const double cp = gas.gamma * gas.R / (gas.gamma - 1.0);
+  const double kappa = cp * mu / gas.Pr_lam;
+  double gr[4][3];  // gr[s][m] = d s / d x_m = sum_l d[l][s] * k[l][m]
+  for (int s = 0; s < 4; ++s)
+    for (int m = 0; m < 3; ++m)
+      gr[s][m] = d[0][s] * k[0 * 3 + m] + d[1][s] * k[1 * 3 + m] + d[2][s] * k[2 * 3 + m];
+  const double divu = gr[0][0] + gr[1][1] + gr[2][2];
+  const double t00 = 2 * mu * (gr[0][0] - divu / 3.0);
+  const double t01 = mu * (gr[0][1] + gr[1][0]);
+  const double t02 = mu * (gr[0][2] + gr[2][0]);
+  const double t11 = 2 * mu * (gr[1][1] - divu / 3.0);
+  const double t12 = mu * (gr[1][2] + gr[2][1]);
+  const double t22 = 2 * mu * (gr[2][2] - divu / 3.0);
+  const double u = pv[1], v = pv[2], w = pv[3];
+  G[0] = t00; G[1] = t01; G[2] = t02; G[3] = t11; G[4] = t12; G[5] = t22;
+  G[6] = u * t00 + v * t01 + w * t02 - (-kappa * gr[3][0]);
+  G[7] = u * t01 + v * t11 + w * t12 - (-kappa * gr[3][1]);
+  G[8] = u * t02 + v * t12 + w * t22 - (-kappa * gr[3][2]);
+}
+
+// T = p / max(rho R, 1e-300)  (rhs/viscous.py:36)
+FD double temperature(const double pv[5], double R) { return pv[4] / fmax(pv[0] * R, 1e-300); }
+
+// Tensor component index of G for (row r in 0..3, column j): rows 0..2 are
+// tau_rj, row 3 is E_j.
+FD int gidx(int r, int j) {
+  if (r == 3) return 6 + j;
+  const int a = r < j ? r : j, b = r < j ? j : r;
+  return a == 0 ? b : (a == 1 ? 2 + b : 5);
+}
+
+}  // namespace famrdp

@@ -1,0 +1,44 @@
+"""Device-resident stepping: the state lives in HBM and advances in place,
+one launch sequence (or one hipGraph replay) per step.  Used by the bench
+and by callers that run many steps; ``step`` above is the functional API."""
+from __future__ import annotations
+
+from .. import _lib as L
+from .._plan import plan_for
+from ..core.types import Block, Config, State
+
+
+def state_to_device(state: State) -> State:
+    """Copy every array of a State onto the current GPU (float64, contiguous)."""
+    from ..core.types import Metrics
+    dev = L.device_f64
+    return State(blocks=tuple(Block(q=dev(b.q), xyz=dev(b.xyz)) for b in state.blocks),
+                 metrics=tuple(Metrics(jac=dev(m.jac), kxyz=dev(m.kxyz), vol=dev(m.vol)) for m in state.metrics),
+                 t=state.t, step=state.step)
+
+
+class Stepper:
+    def __init__(self, state: State, cfg: Config):
+        self.cfg = cfg
+        self.qs = [L.device_f64(b.q).clone() for b in state.blocks]
+        self.kxyz = [L.device_f64(m.kxyz).contiguous() for m in state.metrics]
+        self.vol = [L.device_f64(m.vol).contiguous() for m in state.metrics]
+        self.xyz = [b.xyz for b in state.blocks]
+        self.metrics = state.metrics
+        self.t = state.t
+        self.nstep = state.step
+        dims = [tuple(q.shape[1:]) for q in self.qs]
+        self.plan = plan_for(dims, cfg.topology, cfg.bc_params, cfg.gas, cfg.scheme.limiter, cfg.ghost)
+
+    def advance(self, dt: float, nsteps: int = 1, *, batched: bool = False, graph: bool = False):
+        if graph:
+            self.plan.step(self.qs, self.kxyz, self.vol, dt, batched=batched, graph_steps=nsteps)
+        else:
+            for _ in range(nsteps):
+                self.plan.step(self.qs, self.kxyz, self.vol, dt, batched=batched)
+        self.t += dt * nsteps
+        self.nstep += nsteps
+
+    def state(self) -> State:
+        blocks = tuple(Block(q=q, xyz=x) for q, x in zip(self.qs, self.xyz))
+        return State(blocks=blocks, metrics=self.metrics, t=self.t, step=self.nstep)

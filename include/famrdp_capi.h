@@ -1,0 +1,149 @@
+/* C ABI of the MI355X implementation of famrdp_jax's RHS + TVD-RK3 path.
+ *
+ * Plain C types only: device pointers to float64 arrays in the reference's
+ * layout, sizes, and an opaque stream handle (a hipStream_t passed as void*;
+ * NULL = the default stream).  Every function returns 0 on success or a
+ * negative FAMRDP_E_* code; famrdp_last_error() gives the message of the
+ * last failure on the calling thread.  No function allocates device memory
+ * except famrdp_plan_create (which sizes the plan's workspace once).
+ *
+ * Array layout (famrdp_jax/core/types.py:26-93): q (5, ni, nj, nk), kxyz
+ * (3, 3, ni, nj, nk) with kxyz[l][m] = d xi_l / d x_m, vol and jac
+ * (ni, nj, nk), xyz (3, ni, nj, nk); C order, k unit stride; ni, nj, nk are
+ * padded sizes (interior + 2*ghost).
+ *
+ * Citations are to the reference repository (LucasLu123/famrdp-jax).
+ */
+#ifndef FAMRDP_CAPI_H
+#define FAMRDP_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FAMRDP_OK 0
+#define FAMRDP_E_INVALID (-1)      /* bad argument (Python: ValueError / TypeError)        */
+#define FAMRDP_E_HIP (-2)          /* HIP runtime error                                     */
+#define FAMRDP_E_UNSUPPORTED (-3)  /* e.g. orient != 0 (Python: NotImplementedError)       */
+#define FAMRDP_E_KEY (-4)          /* unknown limiter name (Python: KeyError)               */
+
+/* Limiters: famrdp_jax/physics/reconstruct/muscl2.py:15 (_LIMITERS). */
+#define FAMRDP_LIMITER_MINMOD 0
+#define FAMRDP_LIMITER_VANLEER 1
+
+/* Ghost-cell BC kinds (famrdp_jax/physics/bc/dispatch.py:11-32). */
+#define FAMRDP_BC_WALL_ADIABATIC 0   /* bc/wall.py:30-32                       */
+#define FAMRDP_BC_WALL_ISOTHERMAL 1  /* bc/wall.py:35-58                       */
+#define FAMRDP_BC_MIRROR_NORMAL 2    /* bc/symmetry.py:8-11, bc/wall.py:61-64  */
+#define FAMRDP_BC_FARFIELD 3         /* bc/farfield.py:7-21                    */
+#define FAMRDP_BC_EXTRAPOLATE 4      /* extension: not in the reference        */
+
+/* famrdp_jax/core/types.py:158-167 (GasModel). */
+typedef struct famrdp_gas {
+  double gamma, R, mu0, T0, Ts, Pr_lam;
+} famrdp_gas;
+
+/* One CUT1TO1 copy of famrdp_jax/mesh/halo.py:79-90, listed in the order the
+ * reference visits them (blocks in order, faces in neighbors-dict order).
+ * Faces are famrdp_jax/core/constants.py:55-61 (IMIN=0 .. KMAX=5). */
+typedef struct famrdp_cut {
+  int32_t tgt_block, tgt_face, src_block, src_face, orient;
+} famrdp_cut;
+
+/* One BC face (block index, face, FAMRDP_BC_* kind, T_wall, q_inf). */
+typedef struct famrdp_bc {
+  int32_t block, face, kind, pad_;
+  double T_wall;
+  double q_inf[5];
+} famrdp_bc;
+
+typedef struct famrdp_plan famrdp_plan;
+
+/* Message of the last error on this thread ("" if none). */
+const char* famrdp_last_error(void);
+/* Library / kernel variant identification (static string). */
+const char* famrdp_version(void);
+
+/* ---- pointwise and operator entry points -------------------------------- */
+
+/* famrdp_jax/physics/eos.py:12-17. q, pv: (5, n). */
+int famrdp_cons_to_prim(const double* q, double* pv, int64_t n, double gamma, void* stream);
+/* famrdp_jax/physics/eos.py:5-9. pv, q: (5, n). */
+int famrdp_prim_to_cons(const double* pv, double* q, int64_t n, double gamma, void* stream);
+/* famrdp_jax/physics/eos.py:24-25. */
+int famrdp_sutherland(const double* T, double* mu, int64_t n, double mu0, double T0, double Ts, void* stream);
+/* famrdp_jax/physics/eos.py:20-21. */
+int famrdp_speed_of_sound(const double* rho, const double* p, double* c, int64_t n, double gamma, void* stream);
+/* famrdp_jax/physics/reconstruct/muscl2.py:5-12: out = minmod(a, b) or
+ * vanleer(a, b) elementwise. */
+int famrdp_limiter(const double* a, const double* b, double* out, int64_t n, int32_t limiter, void* stream);
+/* famrdp_jax/physics/reconstruct/muscl2.py:18-37.  pv (nvar, n0, n1, n2);
+ * axis in 1..3; pv_L/pv_R have n_axis-3 along the axis. */
+int famrdp_muscl2(const double* pv, double* pv_L, double* pv_R, int32_t nvar, int32_t n0, int32_t n1, int32_t n2,
+                  int32_t axis, int32_t limiter, void* stream);
+/* famrdp_jax/flux/roe.py:21-85.  UL, UR, F: (5, n); nxyz (3, n), unit normals. */
+int famrdp_roe_flux(const double* UL, const double* UR, const double* nxyz, double* F, int64_t n, double gamma,
+                    double eps_entropy, void* stream);
+/* famrdp_jax/rhs/invscd.py:18-133 (reconstruct/flux names are unused there). */
+int famrdp_invscd_rhs(const double* q, const double* kxyz, const double* vol, double* rhs, int32_t ni, int32_t nj,
+                      int32_t nk, double gamma, int32_t limiter, void* stream);
+/* famrdp_jax/rhs/viscous.py:19-119.  work: caller scratch of 9*ni*nj*nk doubles. */
+int famrdp_viscous_rhs(const double* q, const double* kxyz, double* rhs, int32_t ni, int32_t nj, int32_t nk,
+                       const famrdp_gas* gas, double* work, void* stream);
+/* famrdp_jax/time/rk.py:17-19 combination: out = a*x + b*(y + c*z), or
+ * y + c*z when mode == 1 (x may be NULL then). */
+int famrdp_axpby(double* out, double a, const double* x, double b, const double* y, double c, const double* z,
+                 int64_t n, int32_t mode, void* stream);
+/* famrdp_jax/mesh/metric.py:31-70.  xyz (3, ni, nj, nk) -> jac, kxyz, vol. */
+int famrdp_compute_metrics(const double* xyz, double* jac, double* kxyz, double* vol, int32_t ni, int32_t nj,
+                           int32_t nk, void* stream);
+
+/* ---- plan: one multi-block configuration (Config.topology + gas + scheme) - */
+
+/* dims: nblocks x {ni, nj, nk} (padded).  cuts/bcs as above.  Builds the
+ * halo gather table (final writer per ghost cell, pre-exchange sources), the
+ * per-face BC tables, the tile table and the stage workspace.
+ * Replaces famrdp_jax/solver/step.py:18-56's per-call setup. */
+#define FAMRDP_PLAN_NO_WORKSPACE 1  /* halo / BC only: no stage buffers (famrdp_step is refused) */
+int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, int32_t ncut, const famrdp_cut* cuts,
+                       int32_t nbc, const famrdp_bc* bcs, const famrdp_gas* gas, int32_t limiter, int32_t ghost,
+                       int32_t plan_flags);
+int famrdp_plan_destroy(famrdp_plan* plan);
+
+/* Step variants (flags of famrdp_step). */
+#define FAMRDP_STEP_BATCHED 1   /* one persistent launch per kernel across all blocks */
+
+/* famrdp_jax/solver/step.py:18-56: halo (mesh/halo.py:56), BCs
+ * (physics/bc/dispatch.py:11), then Shu-Osher RK3 on every padded cell,
+ * in place on q[b].  Pointer arrays have nblocks entries (host memory). */
+int famrdp_step(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol, double dt,
+                int32_t flags, void* stream);
+/* nsteps steps replayed from a hipGraph captured for these pointers, dt and
+ * flags (captured on first use, re-captured when any of them changes). */
+int famrdp_step_graph(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
+                      double dt, int32_t flags, int32_t nsteps, void* stream);
+/* Stage timing for measurement: enable=1 starts recording a HIP event pair
+ * around every RK3 stage's kernels of eager famrdp_step calls (on the launch
+ * stream); enable=0 stops, synchronizes and returns the summed stage time (ms)
+ * and the number of stages timed.  No counterpart in the reference (its
+ * benchmarks/bench_step.py:35-51 times whole steps on the host). */
+int famrdp_plan_timing(famrdp_plan* plan, int32_t enable, double* stage_ms, int64_t* stage_count);
+/* famrdp_jax/mesh/halo.py:56-92 alone, in place. */
+int famrdp_halo_exchange(famrdp_plan* plan, double* const* q, void* stream);
+/* famrdp_jax/physics/bc/dispatch.py:11-32 alone, in place. */
+int famrdp_apply_bc(famrdp_plan* plan, double* const* q, void* stream);
+
+/* Build-defined CFL time step over interior cells of one block plus the
+ * positivity monitor (Config.cfl / Config.nan_check, core/types.py:187-189,
+ * unused in the reference).  out (device, 3 doubles) = {dt, max spectral
+ * radius, number of non-finite / non-positive cells}.  work: 2*1024 doubles. */
+int famrdp_cfl_dt(const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj, int32_t nk,
+                  int32_t ghost, const famrdp_gas* gas, double cfl, double* out, double* work, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FAMRDP_CAPI_H */

@@ -1,0 +1,221 @@
+"""GPU parity of the full TVD-RK3 step (halo + BC + 3 stages) against the
+oracle, at the sizes the oracle runs in seconds, plus size-independent
+properties at the full 256^3 size.
+
+Bars (famrdp_jax semantics, FP64): one step from the same state <= 1e-12
+relative L-inf; free-running divergence after 100 steps <= 1e-9.
+"""
+import numpy as np
+import pytest
+import torch
+
+from famrdp_jax import cases
+from famrdp_jax.solver.step import step
+from famrdp_jax.solver.stepper import Stepper
+from oracle import blocks as OB
+from oracle import metric as OM
+from _util import rel_linf, to_dev
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-12
+LONG_TOL = 1e-9
+
+
+def _oracle_args(state, cfg):
+    return ([m.kxyz for m in state.metrics], [m.vol for m in state.metrics])
+
+
+def _ostep(qs, state, cfg, dt):
+    mets = [(m.kxyz, m.vol) for m in state.metrics]
+    return OB.step(qs, mets, list(cfg.topology), cfg.bc_params, cases.gas_dict(cfg.gas), dt)
+
+
+def _gpu_step_from(qs, state, cfg, dt, **kw):
+    from famrdp_jax.core.types import Block, State
+    st = State(blocks=tuple(Block(q=q, xyz=b.xyz) for q, b in zip(qs, state.blocks)), metrics=state.metrics)
+    out = step(to_dev(st), cfg, dt)
+    return [b.q.cpu().numpy() for b in out.blocks]
+
+
+def test_c1_ten_steps_stepwise_and_free_running():
+    print("case: C1 32^3 periodic, 10 steps", flush=True)
+    state, cfg = cases.case_c1(n=32, seed=0)
+    m = state.metrics[0]
+    dt = OM.cfl_dt(state.blocks[0].q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+    ref = [state.blocks[0].q]
+    st = Stepper(to_dev(state), cfg)
+    worst = 0.0
+    for n in range(10):
+        nxt = _ostep(ref, state, cfg, dt)
+        one = _gpu_step_from(ref, state, cfg, dt)
+        worst = max(worst, rel_linf(one[0], nxt[0]))
+        ref = nxt
+        st.advance(dt)
+    torch.cuda.synchronize()
+    free = rel_linf(st.qs[0].cpu().numpy(), ref[0])
+    print(f"C1 worst per-step rel Linf {worst:.3e}, after 10 steps {free:.3e}", flush=True)
+    assert worst <= STEP_TOL
+    assert free <= LONG_TOL
+
+
+def _c2_small(n):
+    q, xyz_ext, cfg = cases.case_c2(n)
+    jac, k, vol = OM.compute_metrics(xyz_ext)
+    m = cases.crop_metrics(jac, k, vol)
+    from famrdp_jax.core.types import Block, State
+    state = State(blocks=(Block(q=q, xyz=cases.crop_xyz(xyz_ext)),), metrics=(m,))
+    dt = OM.cfl_dt(q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+    return state, cfg, dt
+
+
+def test_c2_grid_curvilinear_100_steps():
+    """Config 2's grid and TGV at 24^3 (oracle-sized), 100 steps."""
+    print("case: C2 grid 24^3 curvilinear TGV, 100 steps", flush=True)
+    state, cfg, dt = _c2_small(24)
+    ref = [state.blocks[0].q]
+    st = Stepper(to_dev(state), cfg)
+    worst = 0.0
+    for n in range(100):
+        nxt = _ostep(ref, state, cfg, dt)
+        if n % 10 == 0:
+            worst = max(worst, rel_linf(_gpu_step_from(ref, state, cfg, dt)[0], nxt[0]))
+        ref = nxt
+    st.advance(dt, 100, graph=True)
+    torch.cuda.synchronize()
+    free = rel_linf(st.qs[0].cpu().numpy(), ref[0])
+    print(f"C2-24 worst per-step {worst:.3e}, after 100 steps {free:.3e}", flush=True)
+    assert worst <= STEP_TOL
+    assert free <= LONG_TOL
+
+
+def _small_multiblock():
+    return cases.case_c3(nb_axis=2, lo=8, hi=14, seed=11, p_glue=0.7)
+
+
+def test_multiblock_plain_batched_graph():
+    print("case: C3-like 2x2x2 blocks, 3 steps (plain / batched / graph)", flush=True)
+    state, cfg = _small_multiblock()
+    dt = min(OM.cfl_dt(b.q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5) for b, m in zip(state.blocks, state.metrics))
+    ref = [b.q for b in state.blocks]
+    for _ in range(3):
+        ref = _ostep(ref, state, cfg, dt)
+    outs = {}
+    for mode in ("plain", "batched", "graph", "graph_batched"):
+        st = Stepper(to_dev(state), cfg)
+        st.advance(dt, 3, batched="batched" in mode, graph="graph" in mode)
+        torch.cuda.synchronize()
+        outs[mode] = [q.cpu().numpy() for q in st.qs]
+        err = max(rel_linf(a, b) for a, b in zip(outs[mode], ref))
+        print(f"  {mode}: rel Linf after 3 steps {err:.3e}", flush=True)
+        assert err <= 1e-11
+    for mode in ("batched", "graph", "graph_batched"):
+        for a, b in zip(outs[mode], outs["plain"]):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_c4_wall_farfield_small():
+    print("case: C4-like 16^3 wall/farfield, 5 steps", flush=True)
+    q, xyz_ext, cfg = cases.case_c4(n=16)
+    jac, k, vol = OM.compute_metrics(xyz_ext)
+    m = cases.crop_metrics(jac, k, vol)
+    from famrdp_jax.core.types import Block, State
+    state = State(blocks=(Block(q=q, xyz=cases.crop_xyz(xyz_ext)),), metrics=(m,))
+    dt = OM.cfl_dt(q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+    ref = [q]
+    worst = 0.0
+    for _ in range(5):
+        nxt = _ostep(ref, state, cfg, dt)
+        worst = max(worst, rel_linf(_gpu_step_from(ref, state, cfg, dt)[0], nxt[0]))
+        ref = nxt
+    print(f"C4-16 worst per-step {worst:.3e}", flush=True)
+    assert worst <= STEP_TOL
+
+
+def test_reference_uniform_flow_farfield():
+    """reference tests/solver/test_step.py:7-42 through the GPU step."""
+    print("case: uniform farfield 16x12x10, 3 steps", flush=True)
+    from famrdp_jax.core.constants import BCType, Face
+    from famrdp_jax.core.types import Block, BlockTopology, Config, GasModel, Metrics, SchemeChoice, State
+    g = 1.4
+    rho, u, p = 1.225, 50.0, 101325.0
+    qinf = np.array([rho, rho * u, 0.0, 0.0, p / (g - 1) + 0.5 * rho * u * u])
+    q = np.broadcast_to(qinf[:, None, None, None], (5, 16, 12, 10)).copy()
+    m = cases.cartesian_metrics((16, 12, 10))
+    topo = BlockTopology(0, {f: None for f in Face}, {f: BCType.FARFIELD for f in Face})
+    cfg = Config(gas=GasModel(), scheme=SchemeChoice("muscl2", "roe", 3, "minmod"),
+                 bc_params={f: {"q_inf": qinf} for f in Face}, topology=(topo,), ghost=2, nan_check=False)
+    state = State(blocks=(Block(q=q, xyz=np.zeros((3, 16, 12, 10))),), metrics=(m,))
+    for _ in range(3):
+        state = step(state, cfg, 1e-6)
+    assert state.step == 3 and state.t == pytest.approx(3e-6)
+    np.testing.assert_allclose(state.blocks[0].q[:, 4:-4, 4:-4, 4:-4], q[:, 4:-4, 4:-4, 4:-4], rtol=1e-5, atol=1e-5)
+
+
+def test_determinism_bitwise():
+    print("case: determinism", flush=True)
+    state, cfg = cases.case_c1(n=20, seed=2)
+    outs = []
+    for _ in range(2):
+        st = Stepper(to_dev(state), cfg)
+        st.advance(0.05, 4)
+        torch.cuda.synchronize()
+        outs.append(st.qs[0].cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+@pytest.fixture(scope="module")
+def c2_full():
+    from famrdp_jax.core.types import Block, State
+    from famrdp_jax.mesh.metric import compute_metrics
+    q, xyz_ext, cfg = cases.case_c2(256)
+    m = compute_metrics(torch.from_numpy(xyz_ext).cuda(), ghost=2)
+    sl = (slice(2, -2),) * 3
+    from famrdp_jax.core.types import Metrics
+    mc = Metrics(jac=m.jac[sl].contiguous(), kxyz=m.kxyz[(slice(None), slice(None)) + sl].contiguous(),
+                 vol=m.vol[sl].contiguous())
+    from famrdp_jax.solver.cfl import cfl_dt
+    dt, nbad = cfl_dt(torch.from_numpy(q).cuda(), mc, cfg)
+    assert nbad == 0
+    state = State(blocks=(Block(q=torch.from_numpy(q).cuda(), xyz=torch.zeros(1, dtype=torch.float64).cuda()),),
+                  metrics=(mc,))
+    return state, cfg, dt
+
+
+def test_c2_full_size_center_subbox_vs_oracle(c2_full):
+    """256^3: after one GPU step, a 26^3 sub-box around the centre matches
+    the oracle's three stages on that sub-box (cells >= 7 from its faces are
+    outside the influence of the crop: 3 stages x stencil radius 2)."""
+    print("case: C2 256^3 centre sub-box, 1 step", flush=True)
+    state, cfg, dt = c2_full
+    st = Stepper(state, cfg)
+    c0 = 117
+    sl = (slice(None),) + (slice(c0, c0 + 26),) * 3
+    q_before = st.qs[0][sl].cpu().numpy()
+    msl = (slice(None), slice(None)) + (slice(c0, c0 + 26),) * 3
+    k_sub = state.metrics[0].kxyz[msl].cpu().numpy()
+    v_sub = state.metrics[0].vol[(slice(c0, c0 + 26),) * 3].cpu().numpy()
+    st.advance(dt)
+    torch.cuda.synchronize()
+    got = st.qs[0][sl].cpu().numpy()
+    from famrdp_jax.core.types import BlockTopology
+    iso = BlockTopology(0, {}, {})
+    ref = OB.step([q_before], [(k_sub, v_sub)], [iso], {}, cases.gas_dict(cfg.gas), dt)[0]
+    inner = (slice(None),) + (slice(7, -7),) * 3
+    err = rel_linf(got[inner], ref[inner])
+    print(f"C2-256 sub-box rel Linf {err:.3e}", flush=True)
+    assert err <= STEP_TOL
+
+
+def test_c2_full_size_modes_agree_and_stay_finite(c2_full):
+    print("case: C2 256^3 plain vs batched vs graph, 3 steps", flush=True)
+    state, cfg, dt = c2_full
+    outs = []
+    for kw in (dict(), dict(batched=True), dict(graph=True)):
+        st = Stepper(state, cfg)
+        st.advance(dt, 3, **kw)
+        torch.cuda.synchronize()
+        outs.append(st.qs[0].clone())
+    assert torch.isfinite(outs[0]).all()
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])

@@ -407,8 +407,8 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
   auto cleanup = [&](int code) { famrdp_plan_destroy(p); return code; };
   for (int b = 0; b < nblocks; ++b) {
     const int32_t* d = &dims[3 * b];
-    if (d[0] < 5 || d[1] < 5 || d[2] < 5)
-      return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": each padded size must be >= 5"));
+    if (d[0] < 2 * ghost || d[1] < 2 * ghost || d[2] < 2 * ghost)
+      return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": each padded size must be >= 2*ghost"));
     p->N.push_back((int64_t)d[0] * d[1] * d[2]);
   }
   // workspace and block table

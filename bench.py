@@ -71,7 +71,9 @@ def bench_c2(n, steps, warmup, timing_steps=3):
     state, cfg, dt = setup_c2(n)
     st = Stepper(state, cfg)
     npad = (n + 4) ** 3
-    log(f"C2 {n}^3: dt={dt:.4e}, warmup {warmup}")
+    v, dims, ms = st.autotune(dt)
+    log(f"C2 {n}^3: dt={dt:.4e}; autotuned tile {dims} (variant {v}); stage-1 ms per variant "
+        + " ".join(f"{x:.3f}" for x in ms))
     st.advance(dt, warmup, graph=True)
     _sync()
     t0 = time.perf_counter()
@@ -89,7 +91,7 @@ def bench_c2(n, steps, warmup, timing_steps=3):
     avg_stage_ms = stage_ms / max(nstage, 1)
     bytes_per_stage = npad * BYTES_PER_PADDED_CELL_STEP / 3.0
     achieved = bytes_per_stage / (avg_stage_ms * 1e-3) / 1e9
-    return dict(n=n, npad=npad, dt=dt, steps=steps, wall=wall, ms_per_step=wall * 1e3 / steps,
+    return dict(n=n, npad=npad, dt=dt, steps=steps, wall=wall, ms_per_step=wall * 1e3 / steps, tile=dims,
                 value=n ** 3 * steps / wall, avg_stage_ms=avg_stage_ms, stage_bytes=bytes_per_stage,
                 achieved_gbs=achieved, finite=finite)
 
@@ -134,13 +136,14 @@ def bench_c3(steps=10, warmup=2):
                      ("per_block_graph", dict(batched=False, graph=True)),
                      ("per_block_plain", dict(batched=False, graph=False))):
         st = Stepper(dev, cfg)
+        v, dims, _ = st.autotune(dt, batched=kw["batched"])
         st.advance(dt, warmup, **kw)
         _sync()
         t0 = time.perf_counter()
         st.advance(dt, steps, **kw)
         _sync()
         wall = time.perf_counter() - t0
-        res[name] = dict(ms_per_step=wall * 1e3 / steps, cell_updates_per_s=ncell * steps / wall)
+        res[name] = dict(ms_per_step=wall * 1e3 / steps, cell_updates_per_s=ncell * steps / wall, tile=dims)
         log(f"C3 {name}: {wall * 1e3 / steps:.3f} ms/step")
         del st
         torch.cuda.empty_cache()
@@ -157,6 +160,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 2 GiB float64 buffer 5 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the step would shard by "
@@ -166,6 +171,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        a = torch.ones(2 ** 28, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        _sync()
+        log("copy probe: 5 copies of 2 GiB (read 2 GiB + write 2 GiB each)")
+        return
     from famrdp_jax import _lib
     _lib.load()
     c2 = bench_c2(args.n, args.steps, args.warmup)
@@ -187,7 +200,7 @@ def main():
                config=dict(workload=f"C2 plain step: single-block curvilinear {args.n}^3 TGV "
                                     f"({args.n + 4}^3 padded), periodic, M0.1, Re1600, CFL0.5, hipGraph replay",
                            global_batch=1, seq_len=None, parallelism="single GPU", interior_cells=args.n ** 3,
-                           dt=c2["dt"], finite_after=c2["finite"]),
+                           dt=c2["dt"], finite_after=c2["finite"], tile=c2["tile"]),
                roofline=roofline)
     if not args.no_batched:
         try:

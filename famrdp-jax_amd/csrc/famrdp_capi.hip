@@ -62,6 +62,9 @@ struct famrdp_plan {
   int64_t bc_maxcells[6] = {0, 0, 0, 0, 0, 0};
   Tile* d_tiles = nullptr;
   int ntiles = 0;
+  Tile3* d_tiles3[NUM_TILE_VARIANTS] = {};
+  int ntiles3[NUM_TILE_VARIANTS] = {};
+  int tile_variant = 0;
   int persistent_grid = 0;
   // workspace
   std::vector<double*> q1, q2, G;
@@ -183,13 +186,29 @@ hipEvent_t next_event(famrdp_plan* p) {
 }
 
 int launch_stages(famrdp_plan* p, double dt, int flags, hipStream_t st) {
+  const bool direct = (flags & FAMRDP_STEP_DIRECT) != 0;
+  if (flags & FAMRDP_STEP_BATCHED) {
+    // every batched launch indexes a device tile table: refuse to launch without one
+    if (!p->d_blocks || (direct ? (!p->d_tiles || p->ntiles < 1)
+                                : (!p->d_tiles3[p->tile_variant] || p->ntiles3[p->tile_variant] < 1)))
+      return fail(FAMRDP_E_INVALID, "batched step: tile table missing");
+  }
   for (int stage = 1; stage <= 3; ++stage) {
     const bool tm = p->timing && !p->capturing;
     hipEvent_t e0 = tm ? next_event(p) : nullptr, e1 = tm ? next_event(p) : nullptr;
     if (tm && (!e0 || !e1)) return fail(FAMRDP_E_HIP, "hipEventCreate failed");
     if (tm) HIP_TRY(hipEventRecord(e0, st));
-    if (flags & FAMRDP_STEP_BATCHED) {
-      k_viscous_flux_batched<<<p->persistent_grid, 256, 0, st>>>(p->d_blocks, p->d_tiles, p->ntiles, stage, p->gas);
+    if (!direct && (flags & FAMRDP_STEP_BATCHED)) {
+      const int v = p->tile_variant;
+      launch_fused_batched(v, p->d_blocks, p->d_tiles3[v], p->ntiles3[v], stage, p->gas, p->limiter, dt, st);
+      LAUNCH_CHECK();
+    } else if (!direct) {
+      for (int b = 0; b < p->nblocks; ++b) {
+        launch_fused(p->tile_variant, p->h_blocks[b], stage, p->gas, p->limiter, dt, st);
+        LAUNCH_CHECK();
+      }
+    } else if (flags & FAMRDP_STEP_BATCHED) {
+      k_viscous_flux_batched<false><<<p->persistent_grid, 256, 0, st>>>(p->d_blocks, p->d_tiles, p->ntiles, stage, p->gas);
       LAUNCH_CHECK();
       k_stage_batched<<<p->persistent_grid, 256, 0, st>>>(p->d_blocks, p->d_tiles, p->ntiles, p->gas.gamma,
                                                           p->limiter, dt, stage);
@@ -225,7 +244,7 @@ extern "C" {
 
 const char* famrdp_last_error(void) { return g_last_error.c_str(); }
 
-const char* famrdp_version(void) { return "famrdp-mi355x 0.1 (gfx950, fp64, direct stage kernels)"; }
+const char* famrdp_version(void) { return "famrdp-mi355x 0.4 (gfx950, fp64, fused stage kernel, autotuned tiles)"; }
 
 int famrdp_cons_to_prim(const double* q, double* pv, int64_t n, double gamma, void* stream) {
   if (n < 0 || (n > 0 && (!q || !pv))) return fail(FAMRDP_E_INVALID, "cons_to_prim: bad arguments");
@@ -264,6 +283,14 @@ int famrdp_limiter(const double* a, const double* b, double* out, int64_t n, int
   if (n < 0 || (n > 0 && (!a || !b || !out))) return fail(FAMRDP_E_INVALID, "limiter: bad arguments");
   if (n == 0) return FAMRDP_OK;
   k_limiter<<<nblk(n), 256, 0, S(stream)>>>(a, b, out, n, limiter);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !out_rcp || !out_sqrt))) return fail(FAMRDP_E_INVALID, "fastmath_probe: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_fastmath_probe<<<nblk(n), 256, 0, S(stream)>>>(x, out_rcp, out_sqrt, n);
   LAUNCH_CHECK();
   return FAMRDP_OK;
 }
@@ -379,6 +406,7 @@ int famrdp_plan_destroy(famrdp_plan* p) {
   if (p->d_halo) (void)hipFree(p->d_halo);
   if (p->d_halo_stage) (void)hipFree(p->d_halo_stage);
   if (p->d_tiles) (void)hipFree(p->d_tiles);
+  for (int v = 0; v < NUM_TILE_VARIANTS; ++v) if (p->d_tiles3[v]) (void)hipFree(p->d_tiles3[v]);
   if (p->d_blocks) (void)hipFree(p->d_blocks);
   if (p->h_blocks_pinned) (void)hipHostFree(p->h_blocks_pinned);
   delete p;
@@ -484,6 +512,21 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
   if (hipMalloc(&p->d_tiles, sizeof(Tile) * tiles.size()) != hipSuccess ||
       hipMemcpy(p->d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(FAMRDP_E_HIP, "plan_create: tile table upload failed"));
+  for (int v = 0; v < NUM_TILE_VARIANTS; ++v) {
+    int TI, TJ, TK;
+    tile_dims(v, TI, TJ, TK);
+    std::vector<Tile3> tiles3;
+    for (int b = 0; b < nblocks; ++b) {
+      const int32_t* d = &dims[3 * b];
+      for (int i = 0; i < d[0]; i += TI)
+        for (int j = 0; j < d[1]; j += TJ)
+          for (int k = 0; k < d[2]; k += TK) tiles3.push_back(Tile3{b, i, j, k});
+    }
+    p->ntiles3[v] = (int)tiles3.size();
+    if (hipMalloc(&p->d_tiles3[v], sizeof(Tile3) * tiles3.size()) != hipSuccess ||
+        hipMemcpy(p->d_tiles3[v], tiles3.data(), sizeof(Tile3) * tiles3.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(FAMRDP_E_HIP, "plan_create: tile table upload failed"));
+  }
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
     hipDeviceProp_t prop;
@@ -492,6 +535,57 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
   p->persistent_grid = std::min(p->ntiles, ncu * 8);
   *out = p;
   return FAMRDP_OK;
+}
+
+int famrdp_plan_set_tile(famrdp_plan* p, int32_t variant) {
+  if (!p || variant < 0 || variant >= NUM_TILE_VARIANTS) return fail(FAMRDP_E_INVALID, "set_tile: bad plan or variant");
+  p->tile_variant = variant;
+  if (p->graph_exec) { (void)hipGraphExecDestroy(p->graph_exec); p->graph_exec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+  return FAMRDP_OK;
+}
+
+int famrdp_plan_get_tile(const famrdp_plan* p, int32_t* variant, int32_t* dims3) {
+  if (!p) return fail(FAMRDP_E_INVALID, "get_tile: plan is NULL");
+  if (variant) *variant = p->tile_variant;
+  if (dims3) tile_dims(p->tile_variant, dims3[0], dims3[1], dims3[2]);
+  return FAMRDP_OK;
+}
+
+int famrdp_plan_autotune(famrdp_plan* p, double* const* q, const double* const* kxyz, const double* const* vol,
+                         double dt, int32_t flags, int32_t reps, double* ms_out, void* stream) {
+  if (!p || !kxyz || !vol || reps < 1) return fail(FAMRDP_E_INVALID, "autotune: bad arguments");
+  if (!p->has_workspace) return fail(FAMRDP_E_INVALID, "autotune: plan was created without stage workspace");
+  int rc = bind(p, q, kxyz, vol, S(stream));
+  if (rc) return rc;
+  hipStream_t st = S(stream);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int v = 0; v < NUM_TILE_VARIANTS; ++v) {
+    // stage 1 reads q and writes the q1 workspace only: the state is untouched
+    for (int r = 0; r <= reps; ++r) {
+      if (r == 1) HIP_TRY(hipEventRecord(e0, st));
+      if (flags & FAMRDP_STEP_BATCHED) {
+        launch_fused_batched(v, p->d_blocks, p->d_tiles3[v], p->ntiles3[v], 1, p->gas, p->limiter, dt, st);
+      } else {
+        for (int b = 0; b < p->nblocks; ++b) launch_fused(v, p->h_blocks[b], 1, p->gas, p->limiter, dt, st);
+      }
+      LAUNCH_CHECK();
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    if (ms_out) ms_out[v] = ms;
+    if (ms < best_ms) { best_ms = ms; best = v; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return famrdp_plan_set_tile(p, best);
 }
 
 int famrdp_halo_exchange(famrdp_plan* p, double* const* q, void* stream) {

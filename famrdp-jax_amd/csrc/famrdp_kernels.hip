@@ -22,6 +22,14 @@ __device__ __forceinline__ void load_prim(const double* __restrict__ q, int64_t 
   cons_to_prim(qc, g, pv);
 }
 
+__device__ __forceinline__ void load_prim_fast(const double* __restrict__ q, int64_t N, int64_t c, double g,
+                                               double pv[5]) {
+  double qc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qc[v] = q[v * N + c];
+  cons_to_prim_fast(qc, g, pv);
+}
+
 // ---------------------------------------------------------------------------
 // Unit kernels
 // ---------------------------------------------------------------------------
@@ -102,6 +110,8 @@ __global__ void k_roe_flux(const double* __restrict__ UL, const double* __restri
 // ---------------------------------------------------------------------------
 
 // Viscous flux tensor G at cell c (rhs/viscous.py:30-108, roll wrap).
+// FAST selects the stage-kernel arithmetic (reciprocals, fast sqrt).
+template <bool FAST>
 __device__ __forceinline__ void cell_viscous_flux(const double* __restrict__ q, const double* __restrict__ kxyz,
                                                   double* __restrict__ G, int ni, int nj, int nk, int64_t N,
                                                   int64_t c, const Gas& gas) {
@@ -110,24 +120,30 @@ __device__ __forceinline__ void cell_viscous_flux(const double* __restrict__ q, 
   const int64_t cm[3] = {c + (wrap_m(i, ni) - i) * si, c + (wrap_m(j, nj) - j) * sj, c + (wrap_m(k, nk) - k)};
   const int64_t cp[3] = {c + (wrap_p(i, ni) - i) * si, c + (wrap_p(j, nj) - j) * sj, c + (wrap_p(k, nk) - k)};
   double pv[5];
-  load_prim(q, N, c, gas.gamma, pv);
-  const double T = temperature(pv, gas.R);
+  if (FAST) load_prim_fast(q, N, c, gas.gamma, pv); else load_prim(q, N, c, gas.gamma, pv);
+  const double T = FAST ? temperature_fast(pv, gas.R) : temperature(pv, gas.R);
   double d[3][4];
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     double pp[5], pm[5];
-    load_prim(q, N, cp[l], gas.gamma, pp);
-    load_prim(q, N, cm[l], gas.gamma, pm);
+    if (FAST) {
+      load_prim_fast(q, N, cp[l], gas.gamma, pp);
+      load_prim_fast(q, N, cm[l], gas.gamma, pm);
+    } else {
+      load_prim(q, N, cp[l], gas.gamma, pp);
+      load_prim(q, N, cm[l], gas.gamma, pm);
+    }
     d[l][0] = 0.5 * (pp[1] - pm[1]);
     d[l][1] = 0.5 * (pp[2] - pm[2]);
     d[l][2] = 0.5 * (pp[3] - pm[3]);
-    d[l][3] = 0.5 * (temperature(pp, gas.R) - temperature(pm, gas.R));
+    d[l][3] = FAST ? 0.5 * (temperature_fast(pp, gas.R) - temperature_fast(pm, gas.R))
+                   : 0.5 * (temperature(pp, gas.R) - temperature(pm, gas.R));
   }
   double kk[9];
 #pragma unroll
   for (int m = 0; m < 9; ++m) kk[m] = kxyz[m * N + c];
   double g9[9];
-  viscous_flux(pv, T, d, kk, gas, g9);
+  if (FAST) viscous_flux_fast(pv, T, d, kk, gas, g9); else viscous_flux(pv, T, d, kk, gas, g9);
 #pragma unroll
   for (int m = 0; m < 9; ++m) G[m * N + c] = g9[m];
 }
@@ -228,7 +244,21 @@ __global__ void __launch_bounds__(256) k_viscous_flux(const double* __restrict__
                                                       double* __restrict__ G, int ni, int nj, int nk, Gas gas) {
   const int64_t N = (int64_t)ni * nj * nk;
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (c < N) cell_viscous_flux(q, kxyz, G, ni, nj, nk, N, c, gas);
+  if (c < N) cell_viscous_flux<false>(q, kxyz, G, ni, nj, nk, N, c, gas);
+}
+
+__global__ void __launch_bounds__(256) k_viscous_flux_fast(const double* __restrict__ q,
+                                                           const double* __restrict__ kxyz, double* __restrict__ G,
+                                                           int ni, int nj, int nk, Gas gas) {
+  const int64_t N = (int64_t)ni * nj * nk;
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c < N) cell_viscous_flux<true>(q, kxyz, G, ni, nj, nk, N, c, gas);
+}
+
+__global__ void k_fastmath_probe(const double* __restrict__ x, double* __restrict__ r, double* __restrict__ s,
+                                 int64_t n) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) { r[t] = rcp(x[t]); s[t] = fsqrt(x[t]); }
 }
 
 __global__ void __launch_bounds__(256) k_stage(const double* __restrict__ qin, const double* q0,
@@ -277,6 +307,7 @@ __global__ void k_axpby(double* __restrict__ out, double a, const double* __rest
 // every block of the plan (the block-offset table is `blocks`).
 // ---------------------------------------------------------------------------
 
+template <bool FAST>
 __global__ void __launch_bounds__(256) k_viscous_flux_batched(const BlockDesc* __restrict__ blocks,
                                                               const Tile* __restrict__ tiles, int ntiles, int stage,
                                                               Gas gas) {
@@ -286,7 +317,7 @@ __global__ void __launch_bounds__(256) k_viscous_flux_batched(const BlockDesc* _
     const int64_t c = (int64_t)tl.start + threadIdx.x;
     if (c < b.N) {
       const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
-      cell_viscous_flux(qin, b.kxyz, b.G, b.ni, b.nj, b.nk, b.N, c, gas);
+      cell_viscous_flux<FAST>(qin, b.kxyz, b.G, b.ni, b.nj, b.nk, b.N, c, gas);
     }
   }
 }
@@ -508,6 +539,341 @@ __global__ void __launch_bounds__(256) k_cfl_final(const double* __restrict__ pa
     out[0] = cfl / slam[0];
     out[1] = slam[0];
     out[2] = sbad[0];
+  }
+}
+
+}  // namespace famrdp
+
+// ===========================================================================
+// Fused stage kernel: one launch per RK3 stage, every face flux and every
+// viscous flux tensor computed once per tile.
+//
+// A workgroup owns a TI x TJ x TK tile (one thread per cell).
+//  1. Stage primitives in LDS for the tile, a full one-cell halo box and a
+//     second halo layer along each axis.  Halo cells are read at wrapped
+//     (periodic) block coordinates: that is the reference's jnp.roll for the
+//     viscous terms, and the inviscid faces never reach a wrapped cell (they
+//     exist only between cells 1..n-2 of their axis).
+//  2. Viscous flux tensor G (rhs/viscous.py:30-108) for the tile plus a
+//     one-cell cross halo into LDS; every cell takes its viscous divergence.
+//  3. For each axis in the reference's order (i, j, k): MUSCL-2 + Roe face
+//     fluxes of the tile's low faces plus one layer of high faces into an LDS
+//     face buffer (aliasing G), then -(F(c+1/2) - F(c-1/2)) / vol for cells
+//     with 2 <= c_d <= n_d - 3 (rhs/invscd.py:100-131).
+//  4. rhs = inv + vis and the Shu-Osher combination (solver/step.py:45-53).
+// ===========================================================================
+namespace famrdp {
+
+struct Tile3 {
+  int32_t block, i0, j0, k0;
+};
+
+template <int TI, int TJ, int TK>
+struct FusedGeom {
+  static constexpr int NT = TI * TJ * TK;
+  static constexpr int B0J = TJ + 2, B0K = TK + 2;
+  static constexpr int NR0 = (TI + 2) * B0J * B0K;      // box with a one-cell halo
+  static constexpr int NR1 = 2 * TJ * TK;               // second i layer  [xe][y][z]
+  static constexpr int NR2 = 2 * TI * TK;               // second j layer  [ye][x][z]
+  static constexpr int NR3 = 2 * TI * TJ;               // second k layer  [ze][x][y]
+  static constexpr int NPV = NR0 + NR1 + NR2 + NR3;
+  static constexpr int NG = NT + 2 * (TJ * TK + TI * TK + TI * TJ);   // tile + cross halo 1
+  static constexpr int F0 = (TI + 1) * TJ * TK, F1 = TI * (TJ + 1) * TK, F2 = TI * TJ * (TK + 1);
+  static constexpr int NF = F0 > F1 ? (F0 > F2 ? F0 : F2) : (F1 > F2 ? F1 : F2);
+  static constexpr int NLDS = 5 * NPV + (9 * NG > 5 * NF ? 9 * NG : 5 * NF);
+
+  __device__ static __forceinline__ int pv_slot(int x, int y, int z) {
+    if (x >= -1 && x <= TI && y >= -1 && y <= TJ && z >= -1 && z <= TK) return ((x + 1) * B0J + (y + 1)) * B0K + (z + 1);
+    if (x < -1 || x > TI) return NR0 + ((x < 0 ? 0 : 1) * TJ + y) * TK + z;
+    if (y < -1 || y > TJ) return NR0 + NR1 + ((y < 0 ? 0 : 1) * TI + x) * TK + z;
+    return NR0 + NR1 + NR2 + ((z < 0 ? 0 : 1) * TI + x) * TJ + y;
+  }
+  // local coordinates of LDS primitive slot h (inverse of pv_slot)
+  __device__ static __forceinline__ int3 pv_cell(int h) {
+    if (h < NR0) return make_int3(h / (B0J * B0K) - 1, (h / B0K) % B0J - 1, h % B0K - 1);
+    h -= NR0;
+    if (h < NR1) return make_int3(h / (TJ * TK) ? TI + 1 : -2, (h / TK) % TJ, h % TK);
+    h -= NR1;
+    if (h < NR2) return make_int3((h / TK) % TI, h / (TI * TK) ? TJ + 1 : -2, h % TK);
+    h -= NR2;
+    return make_int3((h / TJ) % TI, h % TJ, h / (TI * TJ) ? TK + 1 : -2);
+  }
+  __device__ static __forceinline__ int g_slot(int x, int y, int z) {
+    if (x >= 0 && x < TI && y >= 0 && y < TJ && z >= 0 && z < TK) return (x * TJ + y) * TK + z;
+    if (x < 0 || x >= TI) return NT + ((x < 0 ? 0 : 1) * TJ + y) * TK + z;
+    if (y < 0 || y >= TJ) return NT + 2 * TJ * TK + ((y < 0 ? 0 : 1) * TI + x) * TK + z;
+    return NT + 2 * TJ * TK + 2 * TI * TK + ((z < 0 ? 0 : 1) * TI + x) * TJ + y;
+  }
+  // local coordinates of G slot h (inverse of g_slot)
+  __device__ static __forceinline__ int3 g_cell(int h) {
+    if (h < NT) return make_int3(h / (TJ * TK), (h / TK) % TJ, h % TK);
+    h -= NT;
+    if (h < 2 * TJ * TK) return make_int3(h / (TJ * TK) ? TI : -1, (h / TK) % TJ, h % TK);
+    h -= 2 * TJ * TK;
+    if (h < 2 * TI * TK) return make_int3((h / TK) % TI, h / (TI * TK) ? TJ : -1, h % TK);
+    h -= 2 * TI * TK;
+    return make_int3((h / TJ) % TI, h % TJ, h / (TI * TJ) ? TK : -1);
+  }
+};
+
+__device__ __forceinline__ int wrap_to(int i, int n) {
+  while (i < 0) i += n;
+  while (i >= n) i -= n;
+  return i;
+}
+
+// XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so map
+// consecutive ids of one XCD onto consecutive tiles (neighbouring tiles share
+// halo lines in that XCD's L2).
+__device__ __forceinline__ int xcd_remap(int wg, int nwg) {
+  constexpr int X = 8;
+  const int q = nwg / X, r = nwg % X;
+  const int xcd = wg % X, local = wg / X;
+  return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+template <int TI, int TJ, int TK>
+__device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int j0, int k0,
+                                                 const double* __restrict__ qin, const double* q0, double* qout,
+                                                 const Gas& gas, int lim, double dt, int stage) {
+  using T = FusedGeom<TI, TJ, TK>;
+  __shared__ double lds[T::NLDS];
+  double* s_pv = lds;                    // [5][NPV]
+  double* s_g = lds + 5 * T::NPV;        // [9][NG]   (phase 2)
+  double* s_f = s_g;                     // [5][NF]   (phase 3, aliases G)
+  const int t = threadIdx.x;
+  const int ni = b.ni, nj = b.nj, nk = b.nk;
+  const int64_t N = b.N;
+  const int64_t si = (int64_t)nj * nk, sj = nk;
+  const double g = gas.gamma;
+  const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
+  const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
+  const bool own = gi < ni && gj < nj && gk < nk;
+  const int64_t c = (int64_t)gi * si + (int64_t)gj * sj + gk;
+  auto gidx_wrapped = [&](int lx, int ly, int lz) -> int64_t {
+    return (int64_t)wrap_to(i0 + lx, ni) * si + (int64_t)wrap_to(j0 + ly, nj) * sj + wrap_to(k0 + lz, nk);
+  };
+
+  // ---- 1. primitives ------------------------------------------------------
+  for (int h = t; h < T::NPV; h += T::NT) {
+    const int3 lc = T::pv_cell(h);
+    const int lx = lc.x, ly = lc.y, lz = lc.z;
+    const int64_t cc = gidx_wrapped(lx, ly, lz);
+    double qq[5], pp[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qq[v] = qin[v * N + cc];
+    cons_to_prim_fast(qq, g, pp);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) s_pv[v * T::NPV + h] = pp[v];
+  }
+  __syncthreads();
+
+  // ---- 2. viscous flux tensor and its divergence -----------------------------
+  double kk[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kk[m] = own ? b.kxyz[m * N + c] : 0.0;
+  for (int h = t; h < T::NG; h += T::NT) {
+    const int3 lc = T::g_cell(h);
+    const int lx = lc.x, ly = lc.y, lz = lc.z;
+    double kc[9];
+    if (h < T::NT) {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) kc[m] = kk[m];
+      if (!own) {
+        const int64_t cc = gidx_wrapped(lx, ly, lz);
+#pragma unroll
+        for (int m = 0; m < 9; ++m) kc[m] = b.kxyz[m * N + cc];
+      }
+    } else {
+      const int64_t cc = gidx_wrapped(lx, ly, lz);
+#pragma unroll
+      for (int m = 0; m < 9; ++m) kc[m] = b.kxyz[m * N + cc];
+    }
+    double pv[5], d[3][4];
+    const int s0 = T::pv_slot(lx, ly, lz);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) pv[v] = s_pv[v * T::NPV + s0];
+    const double Tc = temperature_fast(pv, gas.R);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const int sp = T::pv_slot(lx + (l == 0), ly + (l == 1), lz + (l == 2));
+      const int sm = T::pv_slot(lx - (l == 0), ly - (l == 1), lz - (l == 2));
+      double pp[5], pm[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) { pp[v] = s_pv[v * T::NPV + sp]; pm[v] = s_pv[v * T::NPV + sm]; }
+      d[l][0] = 0.5 * (pp[1] - pm[1]);
+      d[l][1] = 0.5 * (pp[2] - pm[2]);
+      d[l][2] = 0.5 * (pp[3] - pm[3]);
+      d[l][3] = 0.5 * (temperature_fast(pp, gas.R) - temperature_fast(pm, gas.R));
+    }
+    double g9[9];
+    viscous_flux_fast(pv, Tc, d, kc, gas, g9);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) s_g[m * T::NG + h] = g9[m];
+  }
+  __syncthreads();
+  double vis[4] = {0.0, 0.0, 0.0, 0.0};
+  if (own) {
+    const int sp[3] = {T::g_slot(x + 1, y, z), T::g_slot(x, y + 1, z), T::g_slot(x, y, z + 1)};
+    const int sm[3] = {T::g_slot(x - 1, y, z), T::g_slot(x, y - 1, z), T::g_slot(x, y, z - 1)};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double res = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        const double* Gc = s_g + gidx(r, jj) * T::NG;
+        const double d0 = 0.5 * (Gc[sp[0]] - Gc[sm[0]]);
+        const double d1 = 0.5 * (Gc[sp[1]] - Gc[sm[1]]);
+        const double d2 = 0.5 * (Gc[sp[2]] - Gc[sm[2]]);
+        res = res + (d0 * kk[0 * 3 + jj] + d1 * kk[1 * 3 + jj] + d2 * kk[2 * 3 + jj]);
+      }
+      vis[r] = res;
+    }
+  }
+  __syncthreads();   // G space is reused by the face buffer
+
+  // ---- 3. inviscid faces, axis by axis ---------------------------------------
+  double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const double rvol = own ? rcp(b.vol[c] + 1e-300) : 0.0;
+#pragma unroll 1
+  for (int d = 0; d < 3; ++d) {
+    const int Td = d == 0 ? TI : (d == 1 ? TJ : TK);
+    const int loc_d = d == 0 ? x : (d == 1 ? y : z);
+    const int gl_d = d == 0 ? gi : (d == 1 ? gj : gk);
+    const int n_d = d == 0 ? ni : (d == 1 ? nj : nk);
+    const int org_d = d == 0 ? i0 : (d == 1 ? j0 : k0);
+    const int64_t st_d = d == 0 ? si : (d == 1 ? sj : (int64_t)1);
+    const int Pd = T::NT / Td;
+    const int o_own = d == 0 ? (y * TK + z) : (d == 1 ? (x * TK + z) : (x * TJ + y));
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+      int f, o;
+      if (pass == 0) { f = loc_d; o = o_own; }
+      else { if (t >= Pd) break; f = Td; o = t; }
+      int lx, ly, lz;
+      if (d == 0) { lx = f; ly = o / TK; lz = o % TK; }
+      else if (d == 1) { lx = o / TK; ly = f; lz = o % TK; }
+      else { lx = o / TJ; ly = o % TJ; lz = f; }
+      const int ai = i0 + lx, aj = j0 + ly, ak = k0 + lz;
+      const int a = org_d + f - 1;          // cell left of the face (unwrapped)
+      if (ai < ni && aj < nj && ak < nk && a >= 1 && a <= n_d - 3) {
+        double p4[4][5];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int slot = T::pv_slot(lx + (d == 0 ? s4 - 2 : 0), ly + (d == 1 ? s4 - 2 : 0), lz + (d == 2 ? s4 - 2 : 0));
+#pragma unroll
+          for (int v = 0; v < 5; ++v) p4[s4][v] = s_pv[v * T::NPV + slot];
+        }
+        const int64_t cr = (int64_t)ai * si + (int64_t)aj * sj + ak;
+        const int64_t cl = cr - st_d;
+        double kl[3], kr[3];
+        if (pass == 0) {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) kr[m] = d == 0 ? kk[m] : (d == 1 ? kk[3 + m] : kk[6 + m]);
+        } else {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) kr[m] = b.kxyz[(d * 3 + m) * N + cr];
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) kl[m] = b.kxyz[(d * 3 + m) * N + cl];
+        double Fa[5];
+        face_flux_prim(p4[0], p4[1], p4[2], p4[3], kl, kr, g, lim, Fa);
+        const int slot = d == 2 ? o * (TK + 1) + f : f * Pd + o;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) s_f[v * T::NF + slot] = Fa[v];
+      }
+    }
+    __syncthreads();
+    if (own && gl_d >= 2 && gl_d <= n_d - 3) {
+      const int lo = d == 2 ? o_own * (TK + 1) + loc_d : loc_d * Pd + o_own;
+      const int hi = d == 2 ? lo + 1 : lo + Pd;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) inv[v] = inv[v] + (-(s_f[v * T::NF + hi] - s_f[v * T::NF + lo]) * rvol);
+    }
+    __syncthreads();
+  }
+  if (!own) return;
+
+  // ---- 4. rhs and the RK3 combination ----------------------------------------
+  double rhs[5];
+  rhs[0] = inv[0] + 0.0;
+  rhs[1] = inv[1] + vis[0];
+  rhs[2] = inv[2] + vis[1];
+  rhs[3] = inv[3] + vis[2];
+  rhs[4] = inv[4] + vis[3];
+  double qs[5], qq0[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    qs[v] = qin[v * N + c];
+    qq0[v] = stage == 1 ? 0.0 : q0[v * N + c];
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qout[v * N + c] = rk_combine(stage, qq0[v], qs[v], dt, rhs[v]);
+}
+
+// Tile shapes the autotuner chooses from (threads = TI*TJ*TK, one cell each).
+#define FAMRDP_TILE_VARIANTS(X) \
+  X(0, 4, 4, 16)                \
+  X(1, 4, 8, 16)                \
+  X(2, 8, 4, 16)                \
+  X(3, 2, 8, 16)                \
+  X(4, 4, 4, 32)                \
+  X(5, 8, 8, 8)                 \
+  X(6, 2, 4, 32)
+constexpr int NUM_TILE_VARIANTS = 7;
+
+template <int TI, int TJ, int TK>
+__global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused(BlockDesc b, int stage, Gas gas, int lim, double dt,
+                                                              int ntj, int ntk) {
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
+  const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
+  double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
+  stage_tile_fused<TI, TJ, TK>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, lim, dt, stage);
+}
+
+template <int TI, int TJ, int TK>
+__global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused_batched(const BlockDesc* __restrict__ blocks,
+                                                                      const Tile3* __restrict__ tiles, int stage,
+                                                                      Gas gas, int lim, double dt) {
+  const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+  const BlockDesc b = blocks[tl.block];
+  const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
+  double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
+  stage_tile_fused<TI, TJ, TK>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, lim, dt, stage);
+}
+
+inline void tile_dims(int v, int& ti, int& tj, int& tk) {
+  switch (v) {
+#define FAMRDP_TD(id, a, b, c) case id: ti = a; tj = b; tk = c; return;
+    FAMRDP_TILE_VARIANTS(FAMRDP_TD)
+#undef FAMRDP_TD
+    default: ti = 4; tj = 4; tk = 16;
+  }
+}
+
+// One stage of one block (plain launch) with tile variant v.
+inline void launch_fused(int v, const BlockDesc& d, int stage, const Gas& gas, int lim, double dt, hipStream_t st) {
+  int TI, TJ, TK;
+  tile_dims(v, TI, TJ, TK);
+  const int nti = (d.ni + TI - 1) / TI, ntj = (d.nj + TJ - 1) / TJ, ntk = (d.nk + TK - 1) / TK;
+  switch (v) {
+#define FAMRDP_TL(id, a, b, c) \
+  case id: k_stage_fused<a, b, c><<<nti * ntj * ntk, a * b * c, 0, st>>>(d, stage, gas, lim, dt, ntj, ntk); return;
+    FAMRDP_TILE_VARIANTS(FAMRDP_TL)
+#undef FAMRDP_TL
+  }
+}
+
+// One stage of every block in one launch over a tile table built for v.
+inline void launch_fused_batched(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
+                                 const Gas& gas, int lim, double dt, hipStream_t st) {
+  switch (v) {
+#define FAMRDP_TB(id, a, b, c)                                                                          \
+  case id: k_stage_fused_batched<a, b, c><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, lim, dt); \
+    return;
+    FAMRDP_TILE_VARIANTS(FAMRDP_TB)
+#undef FAMRDP_TB
   }
 }
 

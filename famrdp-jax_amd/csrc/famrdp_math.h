@@ -57,6 +57,19 @@ FD double limit(int lim, double a, double b) {
   return lim == LIM_VANLEER ? vanleer(a, b) : minmod(a, b);
 }
 
+// minmod without the sign products: same-sign arguments give the smaller
+// magnitude with a's sign, anything else gives zero (the reference's
+// s * min(|a|,|b|) with s in {-1, 0, 1}; a zero's sign never reaches q).
+FD double minmod_fast(double a, double b) {
+  const double m = fmin(fabs(a), fabs(b));
+  const bool same = (a > 0.0 && b > 0.0) || (a < 0.0 && b < 0.0);
+  return same ? copysign(m, a) : 0.0;
+}
+
+FD double limit_fast(int lim, double a, double b) {
+  return lim == LIM_VANLEER ? vanleer(a, b) : minmod_fast(a, b);
+}
+
 // flux/roe.py:5-18
 FD void physical_flux(const double U[5], const double n[3], double g, double F[5]) {
   const double rho = fmax(U[0], 1e-300);
@@ -154,6 +167,135 @@ FD void face_flux(const double pv0[5], const double pv1[5], const double pv2[5],
   for (int v = 0; v < 5; ++v) Fa[v] = F[v] * nm;
 }
 
+// ---------------------------------------------------------------------------
+// Stage-kernel forms.  Same mathematics as above, arranged for throughput:
+// each divisor is inverted once and multiplied (x * (1/y) rounds at most one
+// ulp away from x / y), and the Roe flux takes the reconstructed primitives
+// directly instead of converting them to conservative variables and back
+// (the reference's prim_to_cons -> roe_flux round trip is the identity up to
+// rounding).
+// ---------------------------------------------------------------------------
+
+#define DF __device__ __forceinline__
+
+// 1/x from the hardware reciprocal seed refined by two Newton-Raphson steps
+// (each squares the relative error), for finite non-zero x.
+DF double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// sqrt(x) from the hardware reciprocal square-root seed: one coupled
+// Newton step on (sqrt, 1/(2 sqrt)) and two residual corrections; x >= 0.
+DF double fsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return x > 0.0 ? g : x;
+}
+
+DF void cons_to_prim_fast(const double q[5], double g, double pv[5]) {
+  const double rho = q[0];
+  const double r = rcp(rho);
+  const double u = q[1] * r, v = q[2] * r, w = q[3] * r;
+  const double ke = 0.5 * (u * u + v * v + w * w);
+  pv[0] = rho; pv[1] = u; pv[2] = v; pv[3] = w;
+  pv[4] = (g - 1.0) * (q[4] - rho * ke);
+}
+
+// flux/roe.py:21-85 on left/right primitives (rho, u, v, w, p), unit normal n.
+DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3], double g, double F[5]) {
+  const double gm1 = g - 1.0;
+  const double rgm1 = rcp(gm1);
+  const double rhoL = fmax(pl[0], 1e-300), rhoR = fmax(pr[0], 1e-300);
+  const double uL = pl[1], vL = pl[2], wL = pl[3], pL = pl[4];
+  const double uR = pr[1], vR = pr[2], wR = pr[3], pR = pr[4];
+  const double EL = pL * rgm1 + rhoL * (0.5 * (uL * uL + vL * vL + wL * wL));
+  const double ER = pR * rgm1 + rhoR * (0.5 * (uR * uR + vR * vR + wR * wR));
+  const double HL = (EL + pL) * rcp(rhoL), HR = (ER + pR) * rcp(rhoR);
+
+  const double sqL = fsqrt(rhoL), sqR = fsqrt(rhoR);
+  const double rs = rcp(fmax(sqL + sqR, 1e-150));
+  const double uh = (sqL * uL + sqR * uR) * rs;
+  const double vh = (sqL * vL + sqR * vR) * rs;
+  const double wh = (sqL * wL + sqR * wR) * rs;
+  const double Hh = (sqL * HL + sqR * HR) * rs;
+  const double q2 = uh * uh + vh * vh + wh * wh;
+  const double c2 = fmax(gm1 * (Hh - 0.5 * q2), 1e-30);
+  const double rc2 = rcp(c2);
+  const double ch = fsqrt(c2);
+  const double unh = uh * n[0] + vh * n[1] + wh * n[2];
+  const double rhoh = sqL * sqR;
+
+  const double eps = 0.1 * (fabs(unh) + ch);
+  const double reps = rcp(eps);
+  const double lam1 = unh - ch, lam5 = unh + ch;
+  const double al1 = fabs(lam1) >= eps ? fabs(lam1) : 0.5 * (lam1 * lam1 * reps + eps);
+  const double al5 = fabs(lam5) >= eps ? fabs(lam5) : 0.5 * (lam5 * lam5 * reps + eps);
+  const double al2 = fabs(unh);
+
+  const double d0 = rhoR - rhoL;
+  const double d1 = rhoR * uR - rhoL * uL, d2 = rhoR * vR - rhoL * vL, d3 = rhoR * wR - rhoL * wL;
+  const double d4 = ER - EL;
+  const double dp = gm1 * (d4 - uh * d1 - vh * d2 - wh * d3 + 0.5 * q2 * d0);
+  const double dun = (d1 * n[0] + d2 * n[1] + d3 * n[2]) - unh * d0;
+  const double rcd = rhoh * ch * dun;
+  const double a1 = (dp - rcd) * (0.5 * rc2);
+  const double a5 = (dp + rcd) * (0.5 * rc2);
+  const double a23 = d0 - dp * rc2;
+
+  const double rrh = rcp(rhoh);
+  const double dut = (d1 - uh * d0) * rrh, dvt_ = (d2 - vh * d0) * rrh, dwt = (d3 - wh * d0) * rrh;
+  const double dunf = dut * n[0] + dvt_ * n[1] + dwt * n[2];
+  const double tx = dut - dunf * n[0], ty = dvt_ - dunf * n[1], tz = dwt - dunf * n[2];
+
+  const double unL = uL * n[0] + vL * n[1] + wL * n[2];
+  const double unR = uR * n[0] + vR * n[1] + wR * n[2];
+  const double w1 = al1 * a1, w5 = al5 * a5, w2 = al2 * a23, w2r = al2 * rhoh;
+  // F = 1/2 (FL + FR) - 1/2 (w1 R1 + w5 R5 + al2 (a23 R2 + dvt))
+  const double cn0 = ch * n[0], cn1 = ch * n[1], cn2 = ch * n[2], cun = ch * unh;
+  F[0] = 0.5 * (rhoL * unL + rhoR * unR) - 0.5 * (w1 + w5 + w2);
+  F[1] = 0.5 * ((rhoL * uL * unL + pL * n[0]) + (rhoR * uR * unR + pR * n[0])) -
+         0.5 * (w1 * (uh - cn0) + w5 * (uh + cn0) + (w2 * uh + w2r * tx));
+  F[2] = 0.5 * ((rhoL * vL * unL + pL * n[1]) + (rhoR * vR * unR + pR * n[1])) -
+         0.5 * (w1 * (vh - cn1) + w5 * (vh + cn1) + (w2 * vh + w2r * ty));
+  F[3] = 0.5 * ((rhoL * wL * unL + pL * n[2]) + (rhoR * wR * unR + pR * n[2])) -
+         0.5 * (w1 * (wh - cn2) + w5 * (wh + cn2) + (w2 * wh + w2r * tz));
+  F[4] = 0.5 * ((EL + pL) * unL + (ER + pR) * unR) -
+         0.5 * (w1 * (Hh - cun) + w5 * (Hh + cun) + (w2 * (0.5 * q2) + w2r * (uh * tx + vh * ty + wh * tz)));
+}
+
+// One face from the primitives of cells a-1, a, a+1, a+2 and kxyz[d, :] at
+// cells a and a+1 (rhs/invscd.py:53-97), on the stage-kernel forms.
+DF void face_flux_prim(const double pv0[5], const double pv1[5], const double pv2[5], const double pv3[5],
+                       const double k0[3], const double k1[3], double g, int lim, double Fa[5]) {
+  double pL[5], pR[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double sL = limit_fast(lim, pv1[v] - pv0[v], pv2[v] - pv1[v]);
+    const double sR = limit_fast(lim, pv2[v] - pv1[v], pv3[v] - pv2[v]);
+    pL[v] = pv1[v] + 0.5 * sL;
+    pR[v] = pv2[v] - 0.5 * sR;
+  }
+  const double nf0 = 0.5 * (k0[0] + k1[0]), nf1 = 0.5 * (k0[1] + k1[1]), nf2 = 0.5 * (k0[2] + k1[2]);
+  const double nm = fsqrt(nf0 * nf0 + nf1 * nf1 + nf2 * nf2) + 1e-300;
+  const double rn = rcp(nm);
+  const double nu[3] = {nf0 * rn, nf1 * rn, nf2 * rn};
+  double F[5];
+  roe_flux_prim(pL, pR, nu, g, F);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Fa[v] = F[v] * nm;
+}
+
 struct Gas {
   double gamma, R, mu0, T0, Ts, Pr_lam;
 };
@@ -184,6 +326,32 @@ FD void viscous_flux(const double pv[5], double T, const double d[3][4], const d
   G[7] = u * t01 + v * t11 + w * t12 - (-kappa * gr[3][1]);
   G[8] = u * t02 + v * t12 + w * t22 - (-kappa * gr[3][2]);
 }
+
+// viscous_flux on the stage-kernel forms (reciprocals, x*sqrt(x) for x^1.5).
+DF void viscous_flux_fast(const double pv[5], double T, const double d[3][4], const double k[9], const Gas& gas,
+                          double G[9]) {
+  const double x = T * rcp(gas.T0);
+  const double mu = gas.mu0 * (x * fsqrt(x)) * (gas.T0 + gas.Ts) * rcp(T + gas.Ts);
+  const double kappa = gas.gamma * gas.R * rcp(gas.gamma - 1.0) * mu * rcp(gas.Pr_lam);
+  double gr[4][3];
+  for (int s = 0; s < 4; ++s)
+    for (int m = 0; m < 3; ++m)
+      gr[s][m] = d[0][s] * k[0 * 3 + m] + d[1][s] * k[1 * 3 + m] + d[2][s] * k[2 * 3 + m];
+  const double divu3 = (gr[0][0] + gr[1][1] + gr[2][2]) * (1.0 / 3.0);
+  const double t00 = 2 * mu * (gr[0][0] - divu3);
+  const double t01 = mu * (gr[0][1] + gr[1][0]);
+  const double t02 = mu * (gr[0][2] + gr[2][0]);
+  const double t11 = 2 * mu * (gr[1][1] - divu3);
+  const double t12 = mu * (gr[1][2] + gr[2][1]);
+  const double t22 = 2 * mu * (gr[2][2] - divu3);
+  const double u = pv[1], v = pv[2], w = pv[3];
+  G[0] = t00; G[1] = t01; G[2] = t02; G[3] = t11; G[4] = t12; G[5] = t22;
+  G[6] = u * t00 + v * t01 + w * t02 + kappa * gr[3][0];
+  G[7] = u * t01 + v * t11 + w * t12 + kappa * gr[3][1];
+  G[8] = u * t02 + v * t12 + w * t22 + kappa * gr[3][2];
+}
+
+DF double temperature_fast(const double pv[5], double R) { return pv[4] * rcp(fmax(pv[0] * R, 1e-300)); }
 
 // T = p / max(rho R, 1e-300)  (rhs/viscous.py:36)
 FD double temperature(const double pv[5], double R) { return pv[4] / fmax(pv[0] * R, 1e-300); }

@@ -45,6 +45,7 @@ SIGNATURES = {
     "famrdp_sutherland": ([vp, vp, i64, f64, f64, f64, vp], i32),
     "famrdp_speed_of_sound": ([vp, vp, vp, i64, f64, vp], i32),
     "famrdp_limiter": ([vp, vp, vp, i64, i32, vp], i32),
+    "famrdp_fastmath_probe": ([vp, vp, vp, i64, vp], i32),
     "famrdp_muscl2": ([vp, vp, vp, i32, i32, i32, i32, i32, i32, vp], i32),
     "famrdp_roe_flux": ([vp, vp, vp, vp, i64, f64, f64, vp], i32),
     "famrdp_invscd_rhs": ([vp, vp, vp, vp, i32, i32, i32, f64, i32, vp], i32),
@@ -56,6 +57,10 @@ SIGNATURES = {
     "famrdp_plan_destroy": ([vp], i32),
     "famrdp_step": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, i32, vp], i32),
     "famrdp_step_graph": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, i32, i32, vp], i32),
+    "famrdp_plan_autotune": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, i32, i32,
+                              ctypes.POINTER(f64), vp], i32),
+    "famrdp_plan_set_tile": ([vp, i32], i32),
+    "famrdp_plan_get_tile": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
     "famrdp_plan_timing": ([vp, i32, ctypes.POINTER(f64), ctypes.POINTER(i64)], i32),
     "famrdp_halo_exchange": ([vp, ctypes.POINTER(vp), vp], i32),
     "famrdp_apply_bc": ([vp, ctypes.POINTER(vp), vp], i32),

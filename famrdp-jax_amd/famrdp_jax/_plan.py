@@ -106,16 +106,37 @@ class Plan:
     def _ptrs(ts):
         return (L.vp * len(ts))(*[L.ptr(t) for t in ts])
 
-    def step(self, qs, kxyzs, vols, dt, *, batched=False, graph_steps=0):
+    def step(self, qs, kxyzs, vols, dt, *, batched=False, graph_steps=0, direct=False):
         """One step in place on the device tensors ``qs`` (or ``graph_steps``
-        steps replayed from a captured hipGraph)."""
-        flags = 1 if batched else 0
+        steps replayed from a captured hipGraph).  ``direct`` selects the
+        reference-order one-thread-per-cell kernels instead of the tiled ones."""
+        flags = (1 if batched else 0) | (2 if direct else 0)
         if graph_steps:
             L.check(self._lib.famrdp_step_graph(self._h, self._ptrs(qs), self._ptrs(kxyzs), self._ptrs(vols),
                                                 float(dt), flags, int(graph_steps), L.stream()))
         else:
             L.check(self._lib.famrdp_step(self._h, self._ptrs(qs), self._ptrs(kxyzs), self._ptrs(vols), float(dt),
                                           flags, L.stream()))
+
+    def autotune(self, qs, kxyzs, vols, dt, *, batched=False, reps=3):
+        """Time every tile variant of the fused stage kernel on these arrays
+        (stage 1 only, the state is not modified) and keep the fastest.
+        Returns (chosen variant, tile dims, per-variant ms)."""
+        n = 7
+        ms = (L.f64 * n)()
+        L.check(self._lib.famrdp_plan_autotune(self._h, self._ptrs(qs), self._ptrs(kxyzs), self._ptrs(vols), float(dt),
+                                               1 if batched else 0, int(reps), ms, L.stream()))
+        v, dims = self.tile()
+        return v, dims, list(ms)
+
+    def tile(self):
+        v = L.i32(0)
+        d = (L.i32 * 3)()
+        L.check(self._lib.famrdp_plan_get_tile(self._h, ctypes.byref(v), d))
+        return v.value, tuple(d)
+
+    def set_tile(self, variant):
+        L.check(self._lib.famrdp_plan_set_tile(self._h, int(variant)))
 
     def timing_start(self):
         L.check(self._lib.famrdp_plan_timing(self._h, 1, None, None))

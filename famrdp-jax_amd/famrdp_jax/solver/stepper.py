@@ -30,14 +30,19 @@ class Stepper:
         dims = [tuple(q.shape[1:]) for q in self.qs]
         self.plan = plan_for(dims, cfg.topology, cfg.bc_params, cfg.gas, cfg.scheme.limiter, cfg.ghost)
 
-    def advance(self, dt: float, nsteps: int = 1, *, batched: bool = False, graph: bool = False):
+    def advance(self, dt: float, nsteps: int = 1, *, batched: bool = False, graph: bool = False,
+                direct: bool = False):
         if graph:
-            self.plan.step(self.qs, self.kxyz, self.vol, dt, batched=batched, graph_steps=nsteps)
+            self.plan.step(self.qs, self.kxyz, self.vol, dt, batched=batched, graph_steps=nsteps, direct=direct)
         else:
             for _ in range(nsteps):
-                self.plan.step(self.qs, self.kxyz, self.vol, dt, batched=batched)
+                self.plan.step(self.qs, self.kxyz, self.vol, dt, batched=batched, direct=direct)
         self.t += dt * nsteps
         self.nstep += nsteps
+
+    def autotune(self, dt: float, *, batched: bool = False, reps: int = 3):
+        """Pick the fused kernel's tile shape for these blocks (state untouched)."""
+        return self.plan.autotune(self.qs, self.kxyz, self.vol, dt, batched=batched, reps=reps)
 
     def state(self) -> State:
         blocks = tuple(Block(q=q, xyz=x) for q, x in zip(self.qs, self.xyz))

@@ -100,6 +100,11 @@ int famrdp_axpby(double* out, double a, const double* x, double b, const double*
 int famrdp_compute_metrics(const double* xyz, double* jac, double* kxyz, double* vol, int32_t ni, int32_t nj,
                            int32_t nk, void* stream);
 
+/* The stage kernels' reciprocal and square root (hardware seed + Newton
+ * refinement) on n values, for verifying their accuracy against IEEE 1/x and
+ * sqrt(x).  No counterpart in the reference. */
+int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, int64_t n, void* stream);
+
 /* ---- plan: one multi-block configuration (Config.topology + gas + scheme) - */
 
 /* dims: nblocks x {ni, nj, nk} (padded).  cuts/bcs as above.  Builds the
@@ -113,7 +118,8 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
 int famrdp_plan_destroy(famrdp_plan* plan);
 
 /* Step variants (flags of famrdp_step). */
-#define FAMRDP_STEP_BATCHED 1   /* one persistent launch per kernel across all blocks */
+#define FAMRDP_STEP_BATCHED 1   /* one launch per kernel across all blocks (block-offset/tile table) */
+#define FAMRDP_STEP_DIRECT 2    /* reference-order direct kernels (one thread per cell, no LDS); for cross-checks */
 
 /* famrdp_jax/solver/step.py:18-56: halo (mesh/halo.py:56), BCs
  * (physics/bc/dispatch.py:11), then Shu-Osher RK3 on every padded cell,
@@ -124,6 +130,17 @@ int famrdp_step(famrdp_plan* plan, double* const* q, const double* const* kxyz, 
  * flags (captured on first use, re-captured when any of them changes). */
 int famrdp_step_graph(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                       double dt, int32_t flags, int32_t nsteps, void* stream);
+/* Tile-shape selection for the fused stage kernel (no counterpart in the
+ * reference).  famrdp_plan_autotune times `reps` stage-1 launches of every
+ * tile variant for these arrays (stage 1 writes only the plan's workspace, so
+ * the state is not modified), writes the per-variant ms to ms_out (may be
+ * NULL; FAMRDP_NUM_TILE_VARIANTS entries) and keeps the fastest. */
+#define FAMRDP_NUM_TILE_VARIANTS 7
+int famrdp_plan_autotune(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
+                         double dt, int32_t flags, int32_t reps, double* ms_out, void* stream);
+int famrdp_plan_set_tile(famrdp_plan* plan, int32_t variant);
+/* variant index and its tile dims {TI, TJ, TK} (cells per workgroup per axis). */
+int famrdp_plan_get_tile(const famrdp_plan* plan, int32_t* variant, int32_t* dims3);
 /* Stage timing for measurement: enable=1 starts recording a HIP event pair
  * around every RK3 stage's kernels of eager famrdp_step calls (on the launch
  * stream); enable=0 stops, synchronizes and returns the summed stage time (ms)

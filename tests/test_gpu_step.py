@@ -114,6 +114,64 @@ def test_multiblock_plain_batched_graph():
             np.testing.assert_array_equal(a, b)
 
 
+def test_direct_kernels_match_oracle_and_tiled():
+    """The reference-order direct kernels (second, independent GPU
+    implementation) against the oracle and the tiled kernels, on the C1
+    workload and on an uneven multi-block case."""
+    print("case: direct vs tiled vs oracle (C1 16^3, 2x2x2 blocks)", flush=True)
+    state, cfg = cases.case_c1(n=16, seed=4)
+    m = state.metrics[0]
+    dt = OM.cfl_dt(state.blocks[0].q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+    ref = _ostep([state.blocks[0].q], state, cfg, dt)[0]
+    for direct in (True, False):
+        st = Stepper(to_dev(state), cfg)
+        st.advance(dt, direct=direct)
+        torch.cuda.synchronize()
+        assert rel_linf(st.qs[0].cpu().numpy(), ref) <= STEP_TOL
+    state, cfg = _small_multiblock()
+    dt = min(OM.cfl_dt(b.q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5) for b, m in zip(state.blocks, state.metrics))
+    outs = []
+    for kw in (dict(direct=True), dict(direct=True, batched=True), dict(), dict(batched=True)):
+        st = Stepper(to_dev(state), cfg)
+        st.advance(dt, 2, **kw)
+        torch.cuda.synchronize()
+        outs.append([q.cpu().numpy() for q in st.qs])
+    for a, b in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(outs[2], outs[3]):
+        np.testing.assert_array_equal(a, b)
+    assert max(rel_linf(a, b) for a, b in zip(outs[2], outs[0])) <= 1e-12
+
+
+def test_tile_variants_bitwise_identical_and_autotune():
+    """Every tile shape of the fused kernel computes each face and viscous
+    tensor with the same inputs and operation order, so all variants give
+    bit-identical states; the autotuner leaves the state untouched."""
+    print("case: tile variants (C1 20^3 and 2x2x2 blocks)", flush=True)
+    for state, cfg in (cases.case_c1(n=20, seed=5), _small_multiblock()):
+        dt = 0.5 * min(OM.cfl_dt(b.q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+                       for b, m in zip(state.blocks, state.metrics))
+        ref = None
+        for v in range(7):
+            st = Stepper(to_dev(state), cfg)
+            st.plan.set_tile(v)
+            st.advance(dt, 2, batched=(v % 2 == 1))
+            torch.cuda.synchronize()
+            out = [q.cpu().numpy() for q in st.qs]
+            if ref is None:
+                ref = out
+            else:
+                for a, b in zip(out, ref):
+                    np.testing.assert_array_equal(a, b)
+        st = Stepper(to_dev(state), cfg)
+        before = [q.clone() for q in st.qs]
+        v, dims, ms = st.autotune(dt, reps=1)
+        assert 0 <= v < 7 and len(ms) == 7 and all(x > 0 for x in ms)
+        for a, b in zip(st.qs, before):
+            assert torch.equal(a, b)
+        st.plan.set_tile(0)
+
+
 def test_c4_wall_farfield_small():
     print("case: C4-like 16^3 wall/farfield, 5 steps", flush=True)
     q, xyz_ext, cfg = cases.case_c4(n=16)

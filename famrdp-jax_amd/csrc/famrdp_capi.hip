@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "famrdp_kernels.hip"
+#include "famrdp_adjoint.hip"
 #include "../../include/famrdp_capi.h"
 
 using namespace famrdp;
@@ -76,6 +77,17 @@ struct famrdp_plan {
   double graph_dt = 0.0;
   int graph_flags = -1;
   bool has_workspace = true;
+  // adjoint workspace
+  bool has_adjoint = false;
+  BlockDesc* d_blocks_adj = nullptr;         // same blocks with q -> qh (recompute buffer)
+  std::vector<BlockDesc> h_blocks_adj;
+  std::vector<double*> adj_bufs;
+  HaloEntry* d_halo_sorted = nullptr;        // halo entries sorted by source cell
+  HaloSrc* d_halo_src = nullptr;
+  int nhalo_src = 0;
+  double* d_halo_S = nullptr;
+  double* d_bc_snap[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* d_bc_part[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // stage timing (HIP events on the launch stream, eager launches only)
   bool timing = false;
   bool capturing = false;
@@ -152,25 +164,45 @@ int bind(famrdp_plan* p, double* const* q, const double* const* kxyz, const doub
     HIP_TRY(hipStreamSynchronize(st));
     std::memcpy(p->h_blocks_pinned, p->h_blocks.data(), sizeof(BlockDesc) * p->nblocks);
     HIP_TRY(hipMemcpyAsync(p->d_blocks, p->h_blocks_pinned, sizeof(BlockDesc) * p->nblocks, hipMemcpyHostToDevice, st));
+    if (p->has_adjoint) {
+      for (int b = 0; b < p->nblocks; ++b) {
+        BlockDesc a = p->h_blocks[b];
+        a.q = p->h_blocks_adj[b].qh;
+        p->h_blocks_adj[b] = a;
+      }
+      HIP_TRY(hipStreamSynchronize(st));
+      std::memcpy(p->h_blocks_pinned, p->h_blocks_adj.data(), sizeof(BlockDesc) * p->nblocks);
+      HIP_TRY(hipMemcpyAsync(p->d_blocks_adj, p->h_blocks_pinned, sizeof(BlockDesc) * p->nblocks,
+                             hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
     p->blocks_uploaded = true;
   }
   return FAMRDP_OK;
 }
 
-int launch_halo(famrdp_plan* p, hipStream_t st) {
+int launch_halo(famrdp_plan* p, hipStream_t st, const BlockDesc* table = nullptr) {
   if (p->nhalo == 0) return FAMRDP_OK;
-  k_halo_gather<<<nblk(p->nhalo), 256, 0, st>>>(p->d_blocks, p->d_halo, p->nhalo, p->d_halo_stage);
+  const BlockDesc* tb = table ? table : p->d_blocks;
+  k_halo_gather<<<nblk(p->nhalo), 256, 0, st>>>(tb, p->d_halo, p->nhalo, p->d_halo_stage);
   LAUNCH_CHECK();
-  k_halo_scatter<<<nblk(p->nhalo), 256, 0, st>>>(p->d_blocks, p->d_halo, p->nhalo, p->d_halo_stage);
+  k_halo_scatter<<<nblk(p->nhalo), 256, 0, st>>>(tb, p->d_halo, p->nhalo, p->d_halo_stage);
   LAUNCH_CHECK();
   return FAMRDP_OK;
 }
 
-int launch_bc(famrdp_plan* p, hipStream_t st) {
+// BC faces in Face order; with `snapshot`, the inner layers of isothermal
+// faces are saved just before each face is applied (for the reverse sweep).
+int launch_bc(famrdp_plan* p, hipStream_t st, const BlockDesc* table = nullptr, bool snapshot = false) {
+  const BlockDesc* tb = table ? table : p->d_blocks;
   for (int f = 0; f < 6; ++f) {
     if (p->nbc[f] == 0) continue;
     dim3 grid(nblk(p->bc_maxcells[f]), (unsigned)p->nbc[f]);
-    k_bc_face<<<grid, 256, 0, st>>>(p->d_blocks, p->d_bc[f], p->ghost, p->gas.gamma, p->gas.R);
+    if (snapshot) {
+      k_bc_snapshot<<<grid, 256, 0, st>>>(tb, p->d_bc[f], p->ghost, p->d_bc_snap[f], p->bc_maxcells[f]);
+      LAUNCH_CHECK();
+    }
+    k_bc_face<<<grid, 256, 0, st>>>(tb, p->d_bc[f], p->ghost, p->gas.gamma, p->gas.R);
     LAUNCH_CHECK();
   }
   return FAMRDP_OK;
@@ -185,43 +217,47 @@ hipEvent_t next_event(famrdp_plan* p) {
   return p->ev_pool[p->ev_used++];
 }
 
-int launch_stages(famrdp_plan* p, double dt, int flags, hipStream_t st) {
+int launch_stages(famrdp_plan* p, double dt, int flags, hipStream_t st, bool adj = false, int nstages = 3) {
   const bool direct = (flags & FAMRDP_STEP_DIRECT) != 0;
+  const BlockDesc* dtab = adj ? p->d_blocks_adj : p->d_blocks;
+  const std::vector<BlockDesc>& htab = adj ? p->h_blocks_adj : p->h_blocks;
   if (flags & FAMRDP_STEP_BATCHED) {
     // every batched launch indexes a device tile table: refuse to launch without one
     if (!p->d_blocks || (direct ? (!p->d_tiles || p->ntiles < 1)
                                 : (!p->d_tiles3[p->tile_variant] || p->ntiles3[p->tile_variant] < 1)))
       return fail(FAMRDP_E_INVALID, "batched step: tile table missing");
   }
-  for (int stage = 1; stage <= 3; ++stage) {
+  for (int stage = 1; stage <= nstages; ++stage) {
     const bool tm = p->timing && !p->capturing;
     hipEvent_t e0 = tm ? next_event(p) : nullptr, e1 = tm ? next_event(p) : nullptr;
     if (tm && (!e0 || !e1)) return fail(FAMRDP_E_HIP, "hipEventCreate failed");
     if (tm) HIP_TRY(hipEventRecord(e0, st));
     if (!direct && (flags & FAMRDP_STEP_BATCHED)) {
       const int v = p->tile_variant;
-      launch_fused_batched(v, p->d_blocks, p->d_tiles3[v], p->ntiles3[v], stage, p->gas, p->limiter, dt, st);
+      launch_fused_batched(v, dtab, p->d_tiles3[v], p->ntiles3[v], stage, p->gas, p->limiter, dt, st);
       LAUNCH_CHECK();
     } else if (!direct) {
       for (int b = 0; b < p->nblocks; ++b) {
-        launch_fused(p->tile_variant, p->h_blocks[b], stage, p->gas, p->limiter, dt, st);
+        launch_fused(p->tile_variant, htab[b], stage, p->gas, p->limiter, dt, st);
         LAUNCH_CHECK();
       }
     } else if (flags & FAMRDP_STEP_BATCHED) {
-      k_viscous_flux_batched<false><<<p->persistent_grid, 256, 0, st>>>(p->d_blocks, p->d_tiles, p->ntiles, stage, p->gas);
+      k_viscous_flux_batched<false><<<p->persistent_grid, 256, 0, st>>>(dtab, p->d_tiles, p->ntiles, stage, p->gas);
       LAUNCH_CHECK();
-      k_stage_batched<<<p->persistent_grid, 256, 0, st>>>(p->d_blocks, p->d_tiles, p->ntiles, p->gas.gamma,
+      k_stage_batched<<<p->persistent_grid, 256, 0, st>>>(dtab, p->d_tiles, p->ntiles, p->gas.gamma,
                                                           p->limiter, dt, stage);
       LAUNCH_CHECK();
     } else {
       for (int b = 0; b < p->nblocks; ++b) {
-        const BlockDesc& d = p->h_blocks[b];
+        const BlockDesc& d = htab[b];
         const double* qin = stage == 1 ? d.q : (stage == 2 ? d.q1 : d.q2);
         double* qout = stage == 1 ? d.q1 : (stage == 2 ? d.q2 : d.q);
-        k_viscous_flux<<<nblk(d.N), 256, 0, st>>>(qin, d.kxyz, d.G, d.ni, d.nj, d.nk, p->gas);
+        Gas gb = p->gas;
+        gb.mu0 = d.mu0;
+        k_viscous_flux<<<nblk(d.N), 256, 0, st>>>(qin, d.kxyz, d.G, d.ni, d.nj, d.nk, gb);
         LAUNCH_CHECK();
         k_stage<<<nblk(d.N), 256, 0, st>>>(qin, d.q, qout, d.kxyz, d.vol, d.G, d.ni, d.nj, d.nk, p->gas.gamma,
-                                           p->limiter, dt, stage);
+                                           p->limiter, dt * d.dt_scale, stage);
         LAUNCH_CHECK();
       }
     }
@@ -399,6 +435,15 @@ int famrdp_plan_destroy(famrdp_plan* p) {
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  for (auto* x : p->adj_bufs) (void)hipFree(x);
+  if (p->d_blocks_adj) (void)hipFree(p->d_blocks_adj);
+  if (p->d_halo_sorted) (void)hipFree(p->d_halo_sorted);
+  if (p->d_halo_src) (void)hipFree(p->d_halo_src);
+  if (p->d_halo_S) (void)hipFree(p->d_halo_S);
+  for (int f = 0; f < 6; ++f) {
+    if (p->d_bc_snap[f]) (void)hipFree(p->d_bc_snap[f]);
+    if (p->d_bc_part[f]) (void)hipFree(p->d_bc_part[f]);
+  }
   for (auto* x : p->q1) (void)hipFree(x);
   for (auto* x : p->q2) (void)hipFree(x);
   for (auto* x : p->G) (void)hipFree(x);
@@ -459,6 +504,31 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     bd.q1 = q1; bd.q2 = q2; bd.G = G;
     bd.ni = dims[3 * b]; bd.nj = dims[3 * b + 1]; bd.nk = dims[3 * b + 2];
     bd.N = p->N[b];
+    bd.mu0 = p->gas.mu0;
+    bd.dt_scale = 1.0;
+  }
+  p->has_adjoint = (plan_flags & FAMRDP_PLAN_ADJOINT) != 0 && p->has_workspace;
+  if (p->has_adjoint) {
+    p->h_blocks_adj = p->h_blocks;
+    for (int b = 0; b < nblocks; ++b) {
+      const int64_t n = p->N[b];
+      const int64_t sizes[6] = {5 * n, 5 * n, 5 * n, 5 * n, 30 * n, 16 * n};
+      double* bufs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+      for (int i = 0; i < 6; ++i) {
+        if (hipMalloc(&bufs[i], sizeof(double) * sizes[i]) != hipSuccess)
+          return cleanup(fail(FAMRDP_E_HIP, "plan_create: adjoint workspace allocation failed"));
+        p->adj_bufs.push_back(bufs[i]);
+        if (hipMemset(bufs[i], 0, sizeof(double) * sizes[i]) != hipSuccess)
+          return cleanup(fail(FAMRDP_E_HIP, "plan_create: adjoint workspace clear failed"));
+      }
+      BlockDesc& a = p->h_blocks_adj[b];
+      a.qh = bufs[0]; a.la = bufs[1]; a.lb = bufs[2]; a.l0 = bufs[3]; a.fs = bufs[4]; a.vs = bufs[5];
+      a.q = a.qh;
+      p->h_blocks[b].qh = a.qh; p->h_blocks[b].la = a.la; p->h_blocks[b].lb = a.lb;
+      p->h_blocks[b].l0 = a.l0; p->h_blocks[b].fs = a.fs; p->h_blocks[b].vs = a.vs;
+    }
+    if (hipMalloc(&p->d_blocks_adj, sizeof(BlockDesc) * nblocks) != hipSuccess)
+      return cleanup(fail(FAMRDP_E_HIP, "plan_create: adjoint block table allocation failed"));
   }
   if (hipMalloc(&p->d_blocks, sizeof(BlockDesc) * nblocks) != hipSuccess ||
       hipHostMalloc(&p->h_blocks_pinned, sizeof(BlockDesc) * nblocks) != hipSuccess)
@@ -503,6 +573,35 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     if (hipMalloc(&p->d_bc[f], sizeof(BcEntry) * uniq.size()) != hipSuccess ||
         hipMemcpy(p->d_bc[f], uniq.data(), sizeof(BcEntry) * uniq.size(), hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(fail(FAMRDP_E_HIP, "plan_create: bc table upload failed"));
+  }
+  if (p->has_adjoint) {
+    if (p->nhalo > 0) {
+      std::vector<HaloEntry> sorted = halo;
+      std::stable_sort(sorted.begin(), sorted.end(), [](const HaloEntry& a, const HaloEntry& b) {
+        return a.sb != b.sb ? a.sb < b.sb : a.sc < b.sc;
+      });
+      std::vector<HaloSrc> us;
+      for (int e = 0; e < (int)sorted.size(); ++e) {
+        if (us.empty() || us.back().sb != sorted[e].sb || us.back().sc != sorted[e].sc)
+          us.push_back(HaloSrc{sorted[e].sb, sorted[e].sc, e, 0});
+        us.back().count++;
+      }
+      p->nhalo_src = (int)us.size();
+      if (hipMalloc(&p->d_halo_sorted, sizeof(HaloEntry) * sorted.size()) != hipSuccess ||
+          hipMalloc(&p->d_halo_src, sizeof(HaloSrc) * us.size()) != hipSuccess ||
+          hipMalloc(&p->d_halo_S, sizeof(double) * 5 * us.size()) != hipSuccess ||
+          hipMemcpy(p->d_halo_sorted, sorted.data(), sizeof(HaloEntry) * sorted.size(), hipMemcpyHostToDevice) !=
+              hipSuccess ||
+          hipMemcpy(p->d_halo_src, us.data(), sizeof(HaloSrc) * us.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(FAMRDP_E_HIP, "plan_create: adjoint halo table failed"));
+    }
+    for (int f = 0; f < 6; ++f) {
+      if (p->nbc[f] == 0) continue;
+      const size_t n = (size_t)p->nbc[f] * 5 * (size_t)p->bc_maxcells[f];
+      if (hipMalloc(&p->d_bc_snap[f], sizeof(double) * n * p->ghost) != hipSuccess ||
+          hipMalloc(&p->d_bc_part[f], sizeof(double) * n) != hipSuccess)
+        return cleanup(fail(FAMRDP_E_HIP, "plan_create: adjoint BC buffers failed"));
+    }
   }
   // tile table for the batched launches
   std::vector<Tile> tiles;
@@ -586,6 +685,94 @@ int famrdp_plan_autotune(famrdp_plan* p, double* const* q, const double* const* 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return famrdp_plan_set_tile(p, best);
+}
+
+int famrdp_plan_set_block_params(famrdp_plan* p, const double* mu0, const double* dt_scale) {
+  if (!p) return fail(FAMRDP_E_INVALID, "set_block_params: plan is NULL");
+  for (int b = 0; b < p->nblocks; ++b) {
+    if (mu0) p->h_blocks[b].mu0 = mu0[b];
+    if (dt_scale) p->h_blocks[b].dt_scale = dt_scale[b];
+    if (p->has_adjoint) {
+      p->h_blocks_adj[b].mu0 = p->h_blocks[b].mu0;
+      p->h_blocks_adj[b].dt_scale = p->h_blocks[b].dt_scale;
+    }
+  }
+  p->blocks_uploaded = false;   // re-uploaded by the next bind
+  if (p->graph_exec) { (void)hipGraphExecDestroy(p->graph_exec); p->graph_exec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+  return FAMRDP_OK;
+}
+
+int famrdp_step_vjp(famrdp_plan* p, double* const* q_start, const double* const* kxyz, const double* const* vol,
+                    double* const* lam, double dt, double* dqinf, int32_t flags, void* stream) {
+  if (!p || !q_start || !kxyz || !vol || !lam) return fail(FAMRDP_E_INVALID, "step_vjp: bad arguments");
+  if (!p->has_adjoint) return fail(FAMRDP_E_INVALID, "step_vjp: plan was created without FAMRDP_PLAN_ADJOINT");
+  for (int b = 0; b < p->nblocks; ++b)
+    if (!kxyz[b] || !vol[b] || !lam[b] || !q_start[b])
+      return fail(FAMRDP_E_INVALID, "step_vjp: NULL pointer for block " + std::to_string(b));
+  hipStream_t st = S(stream);
+  int rc = bind(p, q_start, kxyz, vol, st);
+  if (rc) return rc;
+  // ---- recompute: q0' = BC(halo(q_start)) into qh, then stages 1 and 2
+  for (int b = 0; b < p->nblocks; ++b) {
+    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].qh, q_start[b], 5 * p->N[b]);
+    LAUNCH_CHECK();
+  }
+  rc = launch_halo(p, st, p->d_blocks_adj);
+  if (rc) return rc;
+  rc = launch_bc(p, st, p->d_blocks_adj, true);
+  if (rc) return rc;
+  rc = launch_stages(p, dt, flags, st, true, 2);
+  if (rc) return rc;
+  // ---- reverse the three stages
+  for (int b = 0; b < p->nblocks; ++b) {
+    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].la, lam[b], 5 * p->N[b]);
+    LAUNCH_CHECK();
+  }
+  for (int stage = 3; stage >= 1; --stage) {
+    k_vjp_faces<<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas.gamma, p->limiter, dt);
+    LAUNCH_CHECK();
+    k_vjp_visc<<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, dt);
+    LAUNCH_CHECK();
+    k_vjp_cell<<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, p->limiter);
+    LAUNCH_CHECK();
+  }
+  // ---- BCs reversed (Face order backwards), then the halo copy
+  for (int f = 5; f >= 0; --f) {
+    if (p->nbc[f] == 0) continue;
+    dim3 grid(nblk(p->bc_maxcells[f]), (unsigned)p->nbc[f]);
+    k_bc_face_vjp<<<grid, 256, 0, st>>>(p->d_blocks_adj, p->d_bc[f], p->ghost, p->gas.gamma, p->gas.R,
+                                        p->d_bc_snap[f], p->d_bc_part[f], p->bc_maxcells[f]);
+    LAUNCH_CHECK();
+    k_bc_reduce<<<p->nbc[f], 256, 0, st>>>(p->d_blocks_adj, p->d_bc[f], p->d_bc_part[f], p->bc_maxcells[f], dqinf);
+    LAUNCH_CHECK();
+  }
+  if (p->nhalo > 0) {
+    k_halo_vjp_gather<<<nblk(p->nhalo_src), 256, 0, st>>>(p->d_blocks_adj, p->d_halo_src, p->nhalo_src,
+                                                          p->d_halo_sorted, p->d_halo_S);
+    LAUNCH_CHECK();
+    k_halo_vjp_clear<<<nblk(p->nhalo), 256, 0, st>>>(p->d_blocks_adj, p->d_halo, p->nhalo);
+    LAUNCH_CHECK();
+    k_halo_vjp_apply<<<nblk(p->nhalo_src), 256, 0, st>>>(p->d_blocks_adj, p->d_halo_src, p->nhalo_src, p->d_halo_S);
+    LAUNCH_CHECK();
+  }
+  for (int b = 0; b < p->nblocks; ++b) {
+    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(lam[b], p->h_blocks_adj[b].l0, 5 * p->N[b]);
+    LAUNCH_CHECK();
+  }
+  return FAMRDP_OK;
+}
+
+int famrdp_objective(int32_t kind, const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj,
+                     int32_t nk, int32_t ghost, double gamma, double* J, double* grad, double* work, void* stream) {
+  if ((kind != 0 && kind != 1) || !q || !kxyz || !vol || !J || !grad || !work || ni < 1 || nj < 1 || nk < 1)
+    return fail(FAMRDP_E_INVALID, "objective: bad arguments");
+  const int parts = 1024;
+  k_objective<<<parts, 256, 0, S(stream)>>>(kind, q, kxyz, vol, ni, nj, nk, ghost, gamma, grad, work);
+  LAUNCH_CHECK();
+  k_sum_final<<<1, 256, 0, S(stream)>>>(work, parts, J);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
 }
 
 int famrdp_halo_exchange(famrdp_plan* p, double* const* q, void* stream) {

@@ -21,6 +21,15 @@ struct BlockDesc {
   int32_t ni, nj, nk;
   int32_t pad_;
   int64_t N;            // ni*nj*nk
+  double mu0;           // Sutherland mu0 of this block (ensembles of cases)
+  double dt_scale;      // this block's time step = dt * dt_scale
+  // adjoint workspace (null unless the plan was created for reverse sweeps)
+  double* qh;           // post-halo/BC state recomputed for the reverse sweep (5, N)
+  double* la;           // stage cotangents, ping (5, N)
+  double* lb;           // stage cotangents, pong (5, N)
+  double* l0;           // cotangent of the post-halo/BC state (5, N)
+  double* fs;           // inviscid face reverse scratch (3 axes x 10, N)
+  double* vs;           // viscous reverse scratch (16, N)
 };
 
 // One ghost cell written by the halo exchange: the final writer (dict order)

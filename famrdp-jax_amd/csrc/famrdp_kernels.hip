@@ -313,11 +313,13 @@ __global__ void __launch_bounds__(256) k_viscous_flux_batched(const BlockDesc* _
                                                               Gas gas) {
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
-    const BlockDesc b = blocks[tl.block];
+    const BlockDesc& b = blocks[tl.block];
     const int64_t c = (int64_t)tl.start + threadIdx.x;
     if (c < b.N) {
       const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
-      cell_viscous_flux<FAST>(qin, b.kxyz, b.G, b.ni, b.nj, b.nk, b.N, c, gas);
+      Gas gb = gas;
+      gb.mu0 = b.mu0;
+      cell_viscous_flux<FAST>(qin, b.kxyz, b.G, b.ni, b.nj, b.nk, b.N, c, gb);
     }
   }
 }
@@ -327,12 +329,12 @@ __global__ void __launch_bounds__(256) k_stage_batched(const BlockDesc* __restri
                                                        double dt, int stage) {
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
-    const BlockDesc b = blocks[tl.block];
+    const BlockDesc& b = blocks[tl.block];
     const int64_t c = (int64_t)tl.start + threadIdx.x;
     if (c < b.N) {
       const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
       double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-      cell_stage(qin, b.q, qout, b.kxyz, b.vol, b.G, b.ni, b.nj, b.nk, b.N, c, g, lim, dt, stage);
+      cell_stage(qin, b.q, qout, b.kxyz, b.vol, b.G, b.ni, b.nj, b.nk, b.N, c, g, lim, dt * b.dt_scale, stage);
     }
   }
 }
@@ -646,6 +648,9 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
   const int64_t N = b.N;
   const int64_t si = (int64_t)nj * nk, sj = nk;
   const double g = gas.gamma;
+  Gas gb = gas;
+  gb.mu0 = b.mu0;
+  dt = dt * b.dt_scale;
   const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
   const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
   const bool own = gi < ni && gj < nj && gk < nk;
@@ -693,7 +698,7 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
     const int s0 = T::pv_slot(lx, ly, lz);
 #pragma unroll
     for (int v = 0; v < 5; ++v) pv[v] = s_pv[v * T::NPV + s0];
-    const double Tc = temperature_fast(pv, gas.R);
+    const double Tc = temperature_fast(pv, gb.R);
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       const int sp = T::pv_slot(lx + (l == 0), ly + (l == 1), lz + (l == 2));
@@ -704,10 +709,10 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
       d[l][0] = 0.5 * (pp[1] - pm[1]);
       d[l][1] = 0.5 * (pp[2] - pm[2]);
       d[l][2] = 0.5 * (pp[3] - pm[3]);
-      d[l][3] = 0.5 * (temperature_fast(pp, gas.R) - temperature_fast(pm, gas.R));
+      d[l][3] = 0.5 * (temperature_fast(pp, gb.R) - temperature_fast(pm, gb.R));
     }
     double g9[9];
-    viscous_flux_fast(pv, Tc, d, kc, gas, g9);
+    viscous_flux_fast(pv, Tc, d, kc, gb, g9);
 #pragma unroll
     for (int m = 0; m < 9; ++m) s_g[m * T::NG + h] = g9[m];
   }
@@ -837,7 +842,7 @@ __global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused_batched(const Bloc
                                                                       const Tile3* __restrict__ tiles, int stage,
                                                                       Gas gas, int lim, double dt) {
   const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
-  const BlockDesc b = blocks[tl.block];
+  const BlockDesc& b = blocks[tl.block];
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
   stage_tile_fused<TI, TJ, TK>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, lim, dt, stage);

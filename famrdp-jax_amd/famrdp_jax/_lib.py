@@ -62,6 +62,10 @@ SIGNATURES = {
     "famrdp_plan_set_tile": ([vp, i32], i32),
     "famrdp_plan_get_tile": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
     "famrdp_plan_timing": ([vp, i32, ctypes.POINTER(f64), ctypes.POINTER(i64)], i32),
+    "famrdp_plan_set_block_params": ([vp, ctypes.POINTER(f64), ctypes.POINTER(f64)], i32),
+    "famrdp_step_vjp": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, vp,
+                         i32, vp], i32),
+    "famrdp_objective": ([i32, vp, vp, vp, i32, i32, i32, i32, f64, vp, vp, vp, vp], i32),
     "famrdp_halo_exchange": ([vp, ctypes.POINTER(vp), vp], i32),
     "famrdp_apply_bc": ([vp, ctypes.POINTER(vp), vp], i32),
     "famrdp_cfl_dt": ([vp, vp, vp, i32, i32, i32, i32, ctypes.POINTER(Gas), f64, vp, vp, vp], i32),

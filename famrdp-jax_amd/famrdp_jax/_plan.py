@@ -80,7 +80,7 @@ class Plan:
     """Owns one ``famrdp_plan``: block sizes, halo table, BC tables, tiles and
     the per-block stage workspace (allocated once, on the current GPU)."""
 
-    def __init__(self, dims, cuts, bcs, gas, limiter="minmod", ghost=2, workspace=True):
+    def __init__(self, dims, cuts, bcs, gas, limiter="minmod", ghost=2, workspace=True, adjoint=False):
         L.require_gpu()
         lib = L.load()
         self.nblocks = len(dims)
@@ -92,7 +92,7 @@ class Plan:
         h = L.vp()
         L.check(lib.famrdp_plan_create(ctypes.byref(h), self.nblocks, flat, len(cuts), ccuts, len(bcs), cbcs,
                                        ctypes.byref(g), L.limiter_code(limiter), int(ghost),
-                                       0 if workspace else 1))
+                                       (0 if workspace else 1) | (2 if adjoint else 0)))
         self._h = h
         self._lib = lib
 
@@ -146,6 +146,16 @@ class Plan:
         ms, n = L.f64(0.0), L.i64(0)
         L.check(self._lib.famrdp_plan_timing(self._h, 0, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def step_vjp(self, q_start, kxyzs, vols, lams, dt, dqinf=None, *, batched=False, direct=False):
+        """In place: lams (cotangent of the step's result) -> cotangent of q_start."""
+        flags = (1 if batched else 0) | (2 if direct else 0)
+        L.check(self._lib.famrdp_step_vjp(self._h, self._ptrs(q_start), self._ptrs(kxyzs), self._ptrs(vols),
+                                          self._ptrs(lams), float(dt), L.ptr(dqinf), flags, L.stream()))
+
+    def set_block_params(self, mu0=None, dt_scale=None):
+        arr = lambda x: None if x is None else (L.f64 * self.nblocks)(*[float(v) for v in x])
+        L.check(self._lib.famrdp_plan_set_block_params(self._h, arr(mu0), arr(dt_scale)))
 
     def halo_exchange(self, qs):
         L.check(self._lib.famrdp_halo_exchange(self._h, self._ptrs(qs), L.stream()))

@@ -97,7 +97,11 @@ def _bump(shape_int, ghost, center, sigma, rng_shape=None):
     return np.exp(-r2 / (2.0 * sigma * sigma))
 
 
-def periodic_bump_state(shape_int, *, mach, gamma=1.4, amp=0.05, sigma=4.0, noise=1e-3, seed=0):
+def periodic_bump_state(shape_int, *, mach, gamma=1.4, amp=0.05, sigma=4.0, noise=1e-3, seed=0, p_noise=0.0):
+    """Density bump plus seeded velocity noise; ``p_noise`` > 0 adds relative
+    pressure noise (a uniform pressure puts every pressure slope of the MUSCL
+    limiter at roundoff level, on the limiter's kink, where the derivative of
+    the step is decided by roundoff)."""
     rng = np.random.default_rng(seed)
     center = rng.uniform(0, 1, 3) * np.asarray(shape_int)
     pad = tuple(s + 2 * GHOST for s in shape_int)
@@ -106,6 +110,8 @@ def periodic_bump_state(shape_int, *, mach, gamma=1.4, amp=0.05, sigma=4.0, nois
     v = rng.uniform(-noise, noise, pad)
     w = rng.uniform(-noise, noise, pad)
     p = np.full(pad, 1.0 / gamma)
+    if p_noise:
+        p = p * (1.0 + rng.uniform(-p_noise, p_noise, pad))
     return prim_to_cons_np(rho, u, v, w, p, gamma)
 
 
@@ -114,11 +120,11 @@ def _config(gas, topos, bc_params=None, cfl=0.5):
                   nan_check=False, cfl=cfl)
 
 
-def case_c1(n=32, seed=0):
+def case_c1(n=32, seed=0, p_noise=0.0):
     """Periodic Cartesian n^3 (C1 at n=32).  Returns (state, cfg)."""
     gamma = 1.4
     gas = gas_nondim(gamma=gamma, Pr=0.72, Re=1000.0, U_ref=0.5)
-    q = periodic_bump_state((n, n, n), mach=0.5, gamma=gamma, seed=seed)
+    q = periodic_bump_state((n, n, n), mach=0.5, gamma=gamma, seed=seed, p_noise=p_noise)
     shape = q.shape[1:]
     state = State(blocks=(Block(q=q, xyz=np.zeros((3,) + shape)),), metrics=(cartesian_metrics(shape),))
     return state, _config(gas, [periodic_topology()])
@@ -145,7 +151,7 @@ def case_c2(n=256):
     return q, xyz_ext, _config(gas, [periodic_topology()])
 
 
-def case_c3(nb_axis=4, lo=16, hi=96, seed=883, p_glue=0.85, mach=2.0):
+def case_c3(nb_axis=4, lo=16, hi=96, seed=883, p_glue=0.85, mach=2.0, p_noise=0.0):
     """Multi-block lattice (C3 at the defaults: 64 blocks).  Blocks (a,b,c)
     have interior sizes (NI[a], NJ[b], NK[c]) so glued neighbours share their
     tangential extents (the only shape-valid gluing with orient 0).  Returns
@@ -194,7 +200,10 @@ def case_c3(nb_axis=4, lo=16, hi=96, seed=883, p_glue=0.85, mach=2.0):
                 pad = tuple(s + 2 * GHOST for s in shape_int)
                 prng = np.random.default_rng(seed * 1000 + me)
                 pert = lambda: prng.uniform(-1e-3, 1e-3, pad)
-                q = prim_to_cons_np(1.0 + pert(), mach + pert(), pert(), pert(), np.full(pad, 1.0 / gamma), gamma)
+                p = np.full(pad, 1.0 / gamma)
+                if p_noise:
+                    p = p * (1.0 + prng.uniform(-p_noise, p_noise, pad))
+                q = prim_to_cons_np(1.0 + pert(), mach + pert(), pert(), pert(), p, gamma)
                 blocks.append(Block(q=q, xyz=np.zeros((3,) + pad)))
                 metrics.append(cartesian_metrics(pad))
     bc_params = {f: {"q_inf": qinf, "extrapolate": True} for f in Face}
@@ -203,15 +212,17 @@ def case_c3(nb_axis=4, lo=16, hi=96, seed=883, p_glue=0.85, mach=2.0):
 
 def case_c4(n=96, mach=0.8, Re=5000.0, seed=4):
     """Wall-bounded curvilinear block (C4 at n=96): KMIN isothermal no-slip
-    wall (T_wall = 1 = T_inf), KMAX farfield, periodic in i and j.  Returns
-    (q, xyz_ext, cfg)."""
+    wall (T_wall = 1 = T_inf), KMAX farfield, periodic in i and j; seeded
+    1e-3 perturbations of all five primitives (pressure included, so that the
+    gradient of the step is not decided by roundoff at limiter kinks).
+    Returns (q, xyz_ext, cfg)."""
     gamma = 1.4
     gas = gas_nondim(gamma=gamma, Pr=0.72, Re=Re, U_ref=mach)
     xyz_ext = curvilinear_xyz(n)
     pad = (n + 2 * GHOST,) * 3
     rng = np.random.default_rng(seed)
     pert = lambda: rng.uniform(-1e-3, 1e-3, pad)
-    q = prim_to_cons_np(1.0 + pert(), mach + pert(), pert(), pert(), np.full(pad, 1.0 / gamma), gamma)
+    q = prim_to_cons_np(1.0 + pert(), mach + pert(), pert(), pert(), (1.0 + pert()) / gamma, gamma)
     qinf = freestream(mach, gamma)
     nbrs = {f: (0, _PAIR[f], 0) for f in (Face.IMIN, Face.IMAX, Face.JMIN, Face.JMAX)}
     nbrs[Face.KMIN] = None

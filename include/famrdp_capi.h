@@ -112,6 +112,7 @@ int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, in
  * per-face BC tables, the tile table and the stage workspace.
  * Replaces famrdp_jax/solver/step.py:18-56's per-call setup. */
 #define FAMRDP_PLAN_NO_WORKSPACE 1  /* halo / BC only: no stage buffers (famrdp_step is refused) */
+#define FAMRDP_PLAN_ADJOINT 2       /* also size the reverse-sweep workspace (famrdp_step_vjp) */
 int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, int32_t ncut, const famrdp_cut* cuts,
                        int32_t nbc, const famrdp_bc* bcs, const famrdp_gas* gas, int32_t limiter, int32_t ghost,
                        int32_t plan_flags);
@@ -147,6 +148,32 @@ int famrdp_plan_get_tile(const famrdp_plan* plan, int32_t* variant, int32_t* dim
  * and the number of stages timed.  No counterpart in the reference (its
  * benchmarks/bench_step.py:35-51 times whole steps on the host). */
 int famrdp_plan_timing(famrdp_plan* plan, int32_t enable, double* stage_ms, int64_t* stage_count);
+/* Per-block Sutherland mu0 and time-step scale (block b advances by
+ * dt * dt_scale[b]) for ensembles of independent cases in one plan; either
+ * array may be NULL.  No counterpart in the reference (one GasModel per
+ * Config, core/types.py:158-189). */
+int famrdp_plan_set_block_params(famrdp_plan* plan, const double* mu0, const double* dt_scale);
+
+/* Reverse of one famrdp_step (solver/step.py:18-56 under jax.grad, see
+ * tests/solver/test_grad.py:10-38): given q_start (the state the step started
+ * from, per block) and lam (cotangent of the step's result, per block, device
+ * 5*N doubles), overwrites lam with the cotangent of q_start.  Recomputes
+ * halo, BCs and stages 1-2, reverses the three stages (viscous, Roe, MUSCL),
+ * the BCs in reverse Face order and the halo copy.  dqinf (device, NULL or
+ * nblocks*6*5 doubles) accumulates the cotangent of each (block, face)
+ * farfield state q_inf.  flags as famrdp_step (the recompute uses the same
+ * kernels).  Needs a plan created with FAMRDP_PLAN_ADJOINT. */
+int famrdp_step_vjp(famrdp_plan* plan, double* const* q_start, const double* const* kxyz, const double* const* vol,
+                    double* const* lam, double dt, double* dqinf, int32_t flags, void* stream);
+
+/* Build-defined objectives for the differentiation workloads (the reference
+ * defines none): kind 0 = interior kinetic energy sum 1/2 rho|u|^2 vol,
+ * kind 1 = integrated KMIN wall pressure sum_{i,j} p |grad zeta| vol at
+ * k = ghost.  J (device, 1 double); grad (device, 5*N) = dJ/dq on every cell;
+ * work (device) >= 1024 doubles. */
+int famrdp_objective(int32_t kind, const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj,
+                     int32_t nk, int32_t ghost, double gamma, double* J, double* grad, double* work, void* stream);
+
 /* famrdp_jax/mesh/halo.py:56-92 alone, in place. */
 int famrdp_halo_exchange(famrdp_plan* plan, double* const* q, void* stream);
 /* famrdp_jax/physics/bc/dispatch.py:11-32 alone, in place. */

@@ -1,0 +1,161 @@
+"""GPU adjoint (famrdp_step_vjp + checkpointing) against the autograd oracle
+(torch float64 through the oracle step, JAX derivative conventions) and
+against central finite differences (h = 1e-6).  Bar: 1e-8 relative."""
+import numpy as np
+import pytest
+import torch
+
+from famrdp_jax import cases
+from famrdp_jax.adjoint import OBJ_KINETIC_ENERGY, OBJ_WALL_PRESSURE, Adjoint, mach_derivative, steps_autograd
+from famrdp_jax.core.constants import BCType, Face
+from famrdp_jax.core.types import Block, BlockTopology, Config, GasModel, Metrics, SchemeChoice, State
+from oracle import blocks as OB
+from oracle import metric as OM
+from _util import to_dev
+from _adjoint_util import _c4_small, _oracle_grad
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-8
+
+
+def _rel(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _where(a, b):
+    """Where the largest gradient errors sit: per variable, per i/j/k index."""
+    d = np.abs(np.asarray(a) - np.asarray(b)) / max(np.abs(b).max(), 1e-300)
+    idx = np.unravel_index(np.argmax(d), d.shape)
+    by = {ax: np.round(d.max(axis=tuple(x for x in range(4) if x != ax)), 12).tolist() for ax in (0, 1, 2, 3)}
+    return f"argmax {idx}; by var {by[0]}; by i {by[1]}; by j {by[2]}; by k {by[3]}"
+
+
+@pytest.mark.parametrize("kind", [OBJ_KINETIC_ENERGY, OBJ_WALL_PRESSURE])
+def test_wall_farfield_gradient_vs_autograd(kind):
+    print(f"case: C4-like 12^3 wall/farfield, 3 steps, objective {kind}", flush=True)
+    state, cfg, dt = _c4_small(12)
+    adj = Adjoint(to_dev(state), cfg)
+    J, g, dqinf = adj.gradient(dt, 3, kind, checkpoint_every=2)
+    Jr, gr, qr = _oracle_grad(state, cfg, dt, 3, kind)
+    e = _rel(g[0].cpu().numpy(), gr[0])
+    eq = _rel(dqinf[0, int(Face.KMAX)].cpu().numpy(), qr[int(Face.KMAX)])
+    print(f"  J {J:.12e} vs {Jr:.12e}; dJ/dq0 rel {e:.2e}; dJ/dq_inf rel {eq:.2e}", flush=True)
+    if e > TOL:
+        print("  " + _where(g[0].cpu().numpy(), gr[0]), flush=True)
+    assert abs(J - Jr) <= 1e-12 * abs(Jr)
+    assert e <= TOL and eq <= TOL
+
+
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer"])
+def test_periodic_gradient_vs_autograd(limiter):
+    print(f"case: periodic 10^3 {limiter}, 4 steps, checkpoint 3", flush=True)
+    st, cfg = cases.case_c1(n=10, seed=7, p_noise=1e-3)
+    cfg = Config(gas=cfg.gas, scheme=SchemeChoice("muscl2", "roe", 3, limiter), bc_params={}, topology=cfg.topology,
+                 ghost=2)
+    m = st.metrics[0]
+    dt = OM.cfl_dt(st.blocks[0].q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+    adj = Adjoint(to_dev(st), cfg)
+    J, g, _ = adj.gradient(dt, 4, OBJ_KINETIC_ENERGY, checkpoint_every=3)
+    Jr, gr, _ = _oracle_grad(st, cfg, dt, 4, OBJ_KINETIC_ENERGY)
+    e = _rel(g[0].cpu().numpy(), gr[0])
+    print(f"  dJ/dq0 rel {e:.2e}", flush=True)
+    if e > TOL:
+        print("  " + _where(g[0].cpu().numpy(), gr[0]), flush=True)
+    assert e <= TOL
+
+
+def test_multiblock_batched_gradient_vs_autograd():
+    print("case: 2x2x2 blocks (cut/farfield/extrapolate), 2 steps, batched", flush=True)
+    st, cfg = cases.case_c3(nb_axis=2, lo=6, hi=9, seed=11, p_glue=0.7, p_noise=1e-3)
+    dt = min(OM.cfl_dt(b.q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5) for b, m in zip(st.blocks, st.metrics))
+    adj = Adjoint(to_dev(st), cfg, batched=True)
+    J, g, dqinf = adj.gradient(dt, 2, OBJ_KINETIC_ENERGY, checkpoint_every=10)
+    Jr, gr, qr = _oracle_grad(st, cfg, dt, 2, OBJ_KINETIC_ENERGY)
+    e = max(_rel(a.cpu().numpy(), b) for a, b in zip(g, gr))
+    # the farfield q_inf is shared by every FARFIELD face (bc_params keyed by face)
+    eq = _rel(dqinf[:, int(Face.IMIN)].sum(0).cpu().numpy(), qr[int(Face.IMIN)])
+    print(f"  dJ/dq0 rel {e:.2e}; dJ/dq_inf rel {eq:.2e}", flush=True)
+    assert e <= TOL and eq <= TOL
+
+
+def test_reference_grad_case_finite_nonzero_and_matches():
+    """reference tests/solver/test_grad.py:10-38: uniform farfield flow,
+    J = sum(q after one step); gradient finite, non-zero, and equal to the
+    autograd oracle (v = w = 0 puts u.n = 0 exactly on j/k faces)."""
+    print("case: reference test_grad 12x10x8", flush=True)
+    g = 1.4
+    rho, u, p = 1.225, 50.0, 101325.0
+    qinf = np.array([rho, rho * u, 0.0, 0.0, p / (g - 1) + 0.5 * rho * u * u])
+    q = np.broadcast_to(qinf[:, None, None, None], (5, 12, 10, 8)).copy()
+    m = cases.cartesian_metrics((12, 10, 8))
+    topo = BlockTopology(0, {f: None for f in Face}, {f: BCType.FARFIELD for f in Face})
+    cfg = Config(gas=GasModel(), scheme=SchemeChoice("muscl2", "roe", 3, "minmod"),
+                 bc_params={f: {"q_inf": qinf} for f in Face}, topology=(topo,), ghost=2, nan_check=False)
+    st = State(blocks=(Block(q=q, xyz=np.zeros((3, 12, 10, 8))),), metrics=(m,))
+    adj = Adjoint(to_dev(st), cfg)
+    lam = [torch.ones_like(adj.q0[0])]
+    adj.reverse(lam, 1e-6, 1, {0: [adj.q0[0].clone()]}, 10, None)
+    torch.cuda.synchronize()
+    gq = lam[0].cpu().numpy()
+    assert np.isfinite(gq).all() and (gq != 0).any()
+    _, gr, _ = _oracle_grad(st, cfg, 1e-6, 1, None, seed_all=True)
+    print(f"  grad max {np.abs(gq).max():.4e}; rel vs oracle {_rel(gq, gr[0]):.2e}", flush=True)
+    if _rel(gq, gr[0]) > TOL:
+        print("  " + _where(gq, gr[0]), flush=True)
+    assert _rel(gq, gr[0]) <= TOL
+
+
+def test_central_finite_differences_field_and_mach():
+    """Directional derivative of the wall-pressure J along a random field
+    direction, and dJ/dM_inf of the kinetic-energy J through the farfield
+    state (after 4 steps the farfield barely reaches the wall: the wall J's
+    Mach derivative, ~5e-7, is below what h = 1e-6 resolves), against central
+    differences."""
+    print("case: C4-like 12^3 FD checks (h = 1e-6)", flush=True)
+    state, cfg, dt = _c4_small(12)
+    nsteps = 4
+    dev = to_dev(state)
+    adj = Adjoint(dev, cfg)
+    rng = np.random.default_rng(31)
+    v = torch.from_numpy(rng.normal(size=adj.q0[0].shape) * np.abs(state.blocks[0].q) * 1e-2).cuda()
+    h = 1e-6
+
+    def J_of(kind, q0, qinf=None):
+        c = cfg if qinf is None else Config(gas=cfg.gas, scheme=cfg.scheme, bc_params={
+            Face.KMIN: cfg.bc_params[Face.KMIN], Face.KMAX: {"q_inf": qinf}}, topology=cfg.topology, ghost=2)
+        a = Adjoint(State(blocks=(Block(q=q0, xyz=dev.blocks[0].xyz),), metrics=dev.metrics), c)
+        out = a.forward(dt, nsteps)
+        return a.objective(out, kind)[0]
+
+    _, g, _ = adj.gradient(dt, nsteps, OBJ_WALL_PRESSURE, checkpoint_every=2)
+    fd = (J_of(OBJ_WALL_PRESSURE, adj.q0[0] + h * v) - J_of(OBJ_WALL_PRESSURE, adj.q0[0] - h * v)) / (2 * h)
+    ad = float((g[0] * v).sum())
+    print(f"  field: FD {fd:.12e} adjoint {ad:.12e} rel {abs(fd - ad) / abs(ad):.2e}", flush=True)
+    assert abs(fd - ad) <= 1e-6 * abs(ad)
+    _, _, dqinf = adj.gradient(dt, nsteps, OBJ_KINETIC_ENERGY, checkpoint_every=2)
+    M = 0.8
+    Jp = J_of(OBJ_KINETIC_ENERGY, adj.q0[0], cases.freestream(M + h))
+    Jm = J_of(OBJ_KINETIC_ENERGY, adj.q0[0], cases.freestream(M - h))
+    fdM = (Jp - Jm) / (2 * h)
+    adM = mach_derivative(dqinf, M)
+    print(f"  Mach: FD {fdM:.12e} adjoint {adM:.12e} rel {abs(fdM - adM) / abs(adM):.2e}", flush=True)
+    assert abs(fdM - adM) <= 1e-6 * abs(adM)
+
+
+def test_torch_autograd_function():
+    print("case: torch.autograd.Function wrapper", flush=True)
+    st, cfg = cases.case_c1(n=8, seed=9, p_noise=1e-3)
+    m = st.metrics[0]
+    dt = OM.cfl_dt(st.blocks[0].q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+    adj = Adjoint(to_dev(st), cfg)
+    q0 = adj.q0[0].clone().requires_grad_(True)
+    out = steps_autograd(q0, adj, dt, 3, 2)
+    (out ** 2).sum().backward()
+    qt = torch.tensor(st.blocks[0].q, requires_grad=True)
+    cur = [qt]
+    for _ in range(3):
+        cur = OB.step(cur, [(torch.tensor(m.kxyz), torch.tensor(m.vol))], list(cfg.topology), {},
+                      cases.gas_dict(cfg.gas), dt)
+    (cur[0] ** 2).sum().backward()
+    assert _rel(q0.grad.cpu().numpy(), qt.grad.numpy()) <= TOL

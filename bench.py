@@ -16,6 +16,8 @@ Also reported on the same JSON line:
                 core) on a bounded sample of the same workload
   batched       the 64-block uneven multi-block case: one launch per kernel
                 across all blocks (hipGraph) vs per-block plain launches
+  adjoint       forward + adjoint over forward cost on the wall-bounded case
+                and on the 32-case ensemble
 """
 from __future__ import annotations
 
@@ -46,12 +48,16 @@ def _sync():
 
 
 def setup_c2(n):
-    import torch
     from famrdp_jax import cases
+    return setup_curvilinear(*cases.case_c2(n))
+
+
+def setup_curvilinear(q, xyz_ext, cfg):
+    """Metrics on the GPU from coordinates with 2 extra layers, cropped."""
+    import torch
     from famrdp_jax.core.types import Block, Metrics, State
     from famrdp_jax.mesh.metric import compute_metrics
     from famrdp_jax.solver.cfl import cfl_dt
-    q, xyz_ext, cfg = cases.case_c2(n)
     m = compute_metrics(torch.from_numpy(xyz_ext).cuda(), ghost=2)
     del xyz_ext
     sl = (slice(2, -2),) * 3
@@ -152,6 +158,48 @@ def bench_c3(steps=10, warmup=2):
     return res
 
 
+def _time_adjoint(adj, dt, nsteps, kind, k):
+    adj.forward(dt, 2)
+    _sync()
+    t0 = time.perf_counter()
+    adj.forward(dt, nsteps)
+    _sync()
+    fwd = time.perf_counter() - t0
+    tm = {}
+    adj.gradient(dt, nsteps, kind, checkpoint_every=k, timing=tm)
+    return dict(steps=nsteps, checkpoint_every=k, forward_ms=fwd * 1e3, gradient_ms=tm["total_s"] * 1e3,
+                ratio=tm["total_s"] / fwd)
+
+
+def bench_adjoint():
+    """forward + adjoint cost over forward cost (eager launches both):
+    C4 (96^3 curvilinear wall/farfield, 50 steps, wall-pressure J, checkpoint
+    every 10) and C5 (32 periodic cases in one batched plan with their own
+    mu0 and dt, 30 steps, kinetic-energy J)."""
+    import torch
+    from famrdp_jax import cases
+    from famrdp_jax.adjoint import OBJ_KINETIC_ENERGY, OBJ_WALL_PRESSURE, Adjoint
+    from famrdp_jax.solver.cfl import cfl_dt
+    from famrdp_jax.solver.stepper import state_to_device
+    res = {}
+    state, cfg, dt = setup_curvilinear(*cases.case_c4(96))
+    adj = Adjoint(state, cfg)
+    res["c4"] = _time_adjoint(adj, dt, 50, OBJ_WALL_PRESSURE, 10)
+    log(f"C4 adjoint: {res['c4']}")
+    del adj, state
+    torch.cuda.empty_cache()
+    pairs = cases.case_c5(32)
+    state, cfg, mu0 = cases.ensemble(pairs)
+    dev = state_to_device(state)
+    dts = [cfl_dt(b.q, m, c)[0] for b, m, (_, c) in zip(dev.blocks, dev.metrics, pairs)]
+    adj = Adjoint(dev, cfg, batched=True)
+    adj.set_block_params(mu0, dts)
+    res["c5"] = _time_adjoint(adj, 1.0, 30, OBJ_KINETIC_ENERGY, 10)
+    res["c5"]["cases"] = len(pairs)
+    log(f"C5 adjoint: {res['c5']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +208,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 2 GiB float64 buffer 5 times (counter calibration)")
     args = ap.parse_args()
@@ -207,6 +256,11 @@ def main():
             out["batched"] = bench_c3()
         except Exception as e:  # report, do not hide the headline
             out["batched"] = dict(error=repr(e))
+    if not args.no_adjoint:
+        try:
+            out["adjoint"] = bench_adjoint()
+        except Exception as e:
+            out["adjoint"] = dict(error=repr(e))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

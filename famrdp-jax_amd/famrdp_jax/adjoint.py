@@ -54,6 +54,17 @@ class Adjoint:
             self._step(qs, dt)
         return qs
 
+    def forward_checkpointed(self, qs, dt, nsteps, checkpoint_every):
+        """Advance copies of ``qs``; returns (final states, {step: states})
+        with the states kept every ``checkpoint_every`` steps (step 0 included)."""
+        cur = [q.detach().clone() for q in qs]
+        checkpoints = {0: [q.clone() for q in cur]}
+        for n in range(nsteps):
+            self._step(cur, dt)
+            if (n + 1) % checkpoint_every == 0 and n + 1 < nsteps:
+                checkpoints[n + 1] = [q.clone() for q in cur]
+        return cur, checkpoints
+
     def objective(self, qs, kind, blocks=None):
         """J (float) and dJ/dq (list of device tensors) for a build-defined
         objective summed over ``blocks`` (default: all)."""
@@ -88,12 +99,7 @@ class Adjoint:
         """Returns (J, dJ/dq0 per block, dJ/dq_inf as (nblocks, 6, 5) tensor)."""
         torch = L._torch()
         t0 = time.perf_counter()
-        cur = [q.clone() for q in self.q0]
-        checkpoints = {0: [q.clone() for q in cur]}
-        for n in range(nsteps):
-            self._step(cur, dt)
-            if (n + 1) % checkpoint_every == 0 and n + 1 < nsteps:
-                checkpoints[n + 1] = [q.clone() for q in cur]
+        cur, checkpoints = self.forward_checkpointed(self.q0, dt, nsteps, checkpoint_every)
         J, lam = self.objective(cur, kind, blocks)
         dqinf = torch.zeros(self.nb * 6 * 5, dtype=torch.float64, device="cuda")
         t1 = time.perf_counter()
@@ -120,12 +126,7 @@ def _torch_function():
 
         @staticmethod
         def forward(ctx, q0, adj: Adjoint, dt: float, nsteps: int, checkpoint_every: int = 10):
-            cur = [q0.detach().clone()]
-            ckpts = {0: [cur[0].clone()]}
-            for n in range(nsteps):
-                adj._step(cur, dt)
-                if (n + 1) % checkpoint_every == 0 and n + 1 < nsteps:
-                    ckpts[n + 1] = [cur[0].clone()]
+            cur, ckpts = adj.forward_checkpointed([q0], dt, nsteps, checkpoint_every)
             ctx.adj, ctx.dt, ctx.nsteps, ctx.k, ctx.ckpts = adj, dt, nsteps, checkpoint_every, ckpts
             return cur[0]
 

@@ -238,7 +238,7 @@ def freestream(mach, gamma=1.4):
     return prim_to_cons_np(np.array(1.0), np.array(mach), np.array(0.0), np.array(0.0), np.array(1.0 / gamma), gamma)
 
 
-def case_c5(ncases=32, seed=5, sizes=(48, 64, 80)):
+def case_c5(ncases=32, seed=5, sizes=(48, 64, 80), p_noise=0.0):
     """Ensemble of independent periodic cases (C5).  Returns a list of
     (state, cfg) pairs, each with its own Mach and Reynolds number."""
     rng = np.random.default_rng(seed)
@@ -248,11 +248,37 @@ def case_c5(ncases=32, seed=5, sizes=(48, 64, 80)):
         mach = float(rng.uniform(0.3, 0.9))
         Re = float(math.exp(rng.uniform(math.log(500.0), math.log(5000.0))))
         gas = gas_nondim(gamma=1.4, Pr=0.72, Re=Re, U_ref=mach)
-        q = periodic_bump_state(shape, mach=mach, seed=seed * 1000 + c)
+        q = periodic_bump_state(shape, mach=mach, seed=seed * 1000 + c, p_noise=p_noise)
         pad = q.shape[1:]
         st = State(blocks=(Block(q=q, xyz=np.zeros((3,) + pad)),), metrics=(cartesian_metrics(pad),))
         out.append((st, _config(gas, [periodic_topology()])))
     return out
+
+
+def ensemble(pairs):
+    """Independent single-block cases as one multi-block configuration, for a
+    batched plan: block c keeps its own topology (renumbered to c), BC
+    parameters must agree between cases (they are keyed by face), and the gas
+    of case 0 is shared except for mu0, returned per block (Plan
+    set_block_params).  Returns (state, cfg, mu0 list)."""
+    blocks, metrics, topos, mu0 = [], [], [], []
+    gas0 = pairs[0][1].gas
+    for c, (st, cfg) in enumerate(pairs):
+        if len(st.blocks) != 1:
+            raise ValueError("ensemble: every case must be a single block")
+        g = cfg.gas
+        if (g.gamma, g.R, g.T0, g.Ts, g.Pr_lam) != (gas0.gamma, gas0.R, gas0.T0, gas0.Ts, gas0.Pr_lam):
+            raise ValueError("ensemble: cases may differ only in mu0")
+        t = cfg.topology[0]
+        nbrs = {f: (None if v is None else (c, v[1], v[2])) for f, v in t.neighbors.items()}
+        topos.append(BlockTopology(block_id=c, neighbors=nbrs, bc_type=dict(t.bc_type)))
+        blocks.append(st.blocks[0])
+        metrics.append(st.metrics[0])
+        mu0.append(g.mu0)
+    cfg0 = pairs[0][1]
+    cfg = Config(gas=gas0, scheme=cfg0.scheme, bc_params=cfg0.bc_params, topology=tuple(topos), ghost=cfg0.ghost,
+                 nan_check=False, cfl=cfg0.cfl)
+    return State(blocks=tuple(blocks), metrics=tuple(metrics)), cfg, mu0
 
 
 def interior(a, ghost=GHOST):

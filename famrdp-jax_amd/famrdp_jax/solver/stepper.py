@@ -4,7 +4,7 @@ and by callers that run many steps; ``step`` above is the functional API."""
 from __future__ import annotations
 
 from .. import _lib as L
-from .._plan import plan_for
+from .._plan import Plan, bcs_from_config, cuts_from_topology, plan_for
 from ..core.types import Block, Config, State
 
 
@@ -18,7 +18,11 @@ def state_to_device(state: State) -> State:
 
 
 class Stepper:
-    def __init__(self, state: State, cfg: Config):
+    """``mu0`` / ``dt_scale`` (one value per block) give each block its own
+    Sutherland mu0 and time step dt * dt_scale (ensembles of independent
+    cases in one batched launch); such a stepper owns an uncached plan."""
+
+    def __init__(self, state: State, cfg: Config, *, mu0=None, dt_scale=None):
         self.cfg = cfg
         self.qs = [L.device_f64(b.q).clone() for b in state.blocks]
         self.kxyz = [L.device_f64(m.kxyz).contiguous() for m in state.metrics]
@@ -28,7 +32,12 @@ class Stepper:
         self.t = state.t
         self.nstep = state.step
         dims = [tuple(q.shape[1:]) for q in self.qs]
-        self.plan = plan_for(dims, cfg.topology, cfg.bc_params, cfg.gas, cfg.scheme.limiter, cfg.ghost)
+        if mu0 is None and dt_scale is None:
+            self.plan = plan_for(dims, cfg.topology, cfg.bc_params, cfg.gas, cfg.scheme.limiter, cfg.ghost)
+        else:
+            self.plan = Plan(dims, cuts_from_topology(cfg.topology), bcs_from_config(cfg.topology, cfg.bc_params),
+                             cfg.gas, cfg.scheme.limiter, cfg.ghost)
+            self.plan.set_block_params(mu0, dt_scale)
 
     def advance(self, dt: float, nsteps: int = 1, *, batched: bool = False, graph: bool = False,
                 direct: bool = False):

@@ -46,7 +46,7 @@ def _oracle_grad(state, cfg, dt, nsteps, kind, seed_all=None):
         J = _torch_objective(cur, mets, kind)
     J.backward()
     # faces whose params carry q_inf but are not FARFIELD leave it unused
-    return float(J), [q.grad.numpy() for q in qs], {
+    return float(J.detach()), [q.grad.numpy() for q in qs], {
         f: (t.grad.numpy() if t.grad is not None else np.zeros(5)) for f, t in qinf.items()}
 
 

@@ -76,3 +76,18 @@ def test_product_fails_loudly_without_gpu():
     import numpy as np
     with pytest.raises(RuntimeError):
         cons_to_prim(np.ones((5, 3)), 1.4)
+
+
+def test_jax_bridge_gated():
+    """JAX is optional: the bridge reports whether it can run and refuses
+    loudly when it cannot."""
+    from famrdp_jax.interop import jax_bridge
+    try:
+        import jax  # noqa: F401
+        have = True
+    except ImportError:
+        have = False
+    assert jax_bridge.available() == have
+    if not have:
+        with pytest.raises(ImportError):
+            jax_bridge.make_steps(None, None, 1.0, 1)

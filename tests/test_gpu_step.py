@@ -172,6 +172,38 @@ def test_tile_variants_bitwise_identical_and_autotune():
         st.plan.set_tile(0)
 
 
+def test_c5_ensemble_per_block_mu0_and_dt():
+    """Independent periodic cases with their own Mach, Reynolds number (mu0)
+    and CFL time step, advanced together by one batched plan
+    (set_block_params, dt = 1, dt_scale = dt_case), against the oracle run
+    case by case; the per-block plain and direct paths agree with it."""
+    print("case: C5-like ensemble of 3 cases (sizes 8-12), 3 steps", flush=True)
+    pairs = cases.case_c5(ncases=3, seed=8, sizes=(8, 10, 12))
+    state, cfg, mu0 = cases.ensemble(pairs)
+    dts = [OM.cfl_dt(st.blocks[0].q, st.metrics[0].kxyz, st.metrics[0].vol, cases.gas_dict(c.gas), 0.5)
+           for st, c in pairs]
+    assert len(set(mu0)) == 3 and len(set(dts)) == 3
+    refs = []
+    for (st, c), dt in zip(pairs, dts):
+        q = [st.blocks[0].q]
+        for _ in range(3):
+            q = _ostep(q, st, c, dt)
+        refs.append(q[0])
+    outs = {}
+    for name, kw in (("batched", dict(batched=True)), ("plain", dict()), ("graph_batched", dict(batched=True, graph=True)),
+                     ("direct_batched", dict(batched=True, direct=True))):
+        stp = Stepper(to_dev(state), cfg, mu0=mu0, dt_scale=dts)
+        stp.advance(1.0, 3, **kw)
+        torch.cuda.synchronize()
+        outs[name] = [q.cpu().numpy() for q in stp.qs]
+        err = max(rel_linf(a, b) for a, b in zip(outs[name], refs))
+        print(f"  {name}: rel Linf after 3 steps {err:.3e}", flush=True)
+        assert err <= 1e-11
+    for name in ("plain", "graph_batched"):
+        for a, b in zip(outs[name], outs["batched"]):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_c4_wall_farfield_small():
     print("case: C4-like 16^3 wall/farfield, 5 steps", flush=True)
     q, xyz_ext, cfg = cases.case_c4(n=16)

@@ -816,17 +816,6 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
   for (int v = 0; v < 5; ++v) qout[v * N + c] = rk_combine(stage, qq0[v], qs[v], dt, rhs[v]);
 }
 
-// Tile shapes the autotuner chooses from (threads = TI*TJ*TK, one cell each).
-#define FAMRDP_TILE_VARIANTS(X) \
-  X(0, 4, 4, 16)                \
-  X(1, 4, 8, 16)                \
-  X(2, 8, 4, 16)                \
-  X(3, 2, 8, 16)                \
-  X(4, 4, 4, 32)                \
-  X(5, 8, 8, 8)                 \
-  X(6, 2, 4, 32)
-constexpr int NUM_TILE_VARIANTS = 7;
-
 template <int TI, int TJ, int TK>
 __global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused(BlockDesc b, int stage, Gas gas, int lim, double dt,
                                                               int ntj, int ntk) {
@@ -848,16 +837,314 @@ __global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused_batched(const Bloc
   stage_tile_fused<TI, TJ, TK>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, lim, dt, stage);
 }
 
+// ===========================================================================
+// Split stage: two launches per RK3 stage (tile tables as the fused kernel).
+//
+//  k_visc_tensor  u, v, w, T of the tile and a one-cell cross halo staged in
+//                 LDS; the viscous flux tensor G (rhs/viscous.py:30-108) of
+//                 every cell, once, written to the plan's G buffer (9, N).
+//  k_stage_split  primitives of the tile and a two-cell cross halo in LDS;
+//                 divergence of G at the cell's six (wrapped) neighbours read
+//                 from the G buffer; MUSCL-2 + Roe faces once per tile as in
+//                 the fused kernel; the Shu-Osher combination.
+// Every value is produced by the same expressions, on the same inputs, in
+// the same order as in the fused kernel, so both kernels give the same bits;
+// the split form holds no viscous tensor in LDS (43-74 KB per workgroup
+// instead of 75-143 KB), so more workgroups share a CU.
+// ===========================================================================
+template <int TI, int TJ, int TK>
+struct SplitGeom {
+  static constexpr int NT = TI * TJ * TK;
+  static constexpr int A0 = TJ * TK, A1 = TI * TK, A2 = TI * TJ;
+  static constexpr int NP = NT + 4 * (A0 + A1 + A2);    // tile + two halo layers per side, per axis
+  static constexpr int F0 = (TI + 1) * TJ * TK, F1 = TI * (TJ + 1) * TK, F2 = TI * TJ * (TK + 1);
+  static constexpr int NF = F0 > F1 ? (F0 > F2 ? F0 : F2) : (F1 > F2 ? F1 : F2);
+  static constexpr int NLDS = 5 * NP + 5 * NF;
+
+  // slot of a cell that lies in the tile or on the halo layers of one axis
+  __device__ static __forceinline__ int p_slot(int x, int y, int z) {
+    if (x < 0 || x >= TI) return NT + ((x < 0 ? x + 2 : x - TI + 2) * TJ + y) * TK + z;
+    if (y < 0 || y >= TJ) return NT + 4 * A0 + ((y < 0 ? y + 2 : y - TJ + 2) * TI + x) * TK + z;
+    if (z < 0 || z >= TK) return NT + 4 * (A0 + A1) + ((z < 0 ? z + 2 : z - TK + 2) * TI + x) * TJ + y;
+    return (x * TJ + y) * TK + z;
+  }
+  // inverse of p_slot
+  __device__ static __forceinline__ int3 p_cell(int h) {
+    if (h < NT) return make_int3(h / (TJ * TK), (h / TK) % TJ, h % TK);
+    h -= NT;
+    if (h < 4 * A0) {
+      const int l = h / A0, r = h % A0;
+      return make_int3(l < 2 ? l - 2 : TI + l - 2, r / TK, r % TK);
+    }
+    h -= 4 * A0;
+    if (h < 4 * A1) {
+      const int l = h / A1, r = h % A1;
+      return make_int3(r / TK, l < 2 ? l - 2 : TJ + l - 2, r % TK);
+    }
+    h -= 4 * A1;
+    const int l = h / A2, r = h % A2;
+    return make_int3(r / TJ, r % TJ, l < 2 ? l - 2 : TK + l - 2);
+  }
+};
+
+template <int TI, int TJ, int TK>
+__device__ __forceinline__ void visc_tile(const BlockDesc& b, int i0, int j0, int k0, const double* __restrict__ qin,
+                                          const Gas& gas) {
+  using T = FusedGeom<TI, TJ, TK>;       // g_slot / g_cell: tile + one-cell cross halo
+  __shared__ double s_w[4 * T::NG];      // u, v, w, T
+  const int t = threadIdx.x;
+  const int ni = b.ni, nj = b.nj, nk = b.nk;
+  const int64_t N = b.N;
+  const int64_t si = (int64_t)nj * nk, sj = nk;
+  Gas gb = gas;
+  gb.mu0 = b.mu0;
+  for (int h = t; h < T::NG; h += T::NT) {
+    const int3 lc = T::g_cell(h);
+    const int64_t cc =
+        (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
+    double qq[5], pp[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qq[v] = qin[v * N + cc];
+    cons_to_prim_fast(qq, gas.gamma, pp);
+    s_w[h] = pp[1];
+    s_w[T::NG + h] = pp[2];
+    s_w[2 * T::NG + h] = pp[3];
+    s_w[3 * T::NG + h] = temperature_fast(pp, gb.R);
+  }
+  __syncthreads();
+  const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
+  const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
+  if (gi >= ni || gj >= nj || gk >= nk) return;
+  const int64_t c = (int64_t)gi * si + (int64_t)gj * sj + gk;
+  const int s0 = T::g_slot(x, y, z);
+  const double pv[5] = {0.0, s_w[s0], s_w[T::NG + s0], s_w[2 * T::NG + s0], 0.0};
+  const double Tc = s_w[3 * T::NG + s0];
+  double d[3][4];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const int sp = T::g_slot(x + (l == 0), y + (l == 1), z + (l == 2));
+    const int sm = T::g_slot(x - (l == 0), y - (l == 1), z - (l == 2));
+#pragma unroll
+    for (int v = 0; v < 4; ++v) d[l][v] = 0.5 * (s_w[v * T::NG + sp] - s_w[v * T::NG + sm]);
+  }
+  double kc[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kc[m] = b.kxyz[m * N + c];
+  double g9[9];
+  viscous_flux_fast(pv, Tc, d, kc, gb, g9);
+#pragma unroll
+  for (int m = 0; m < 9; ++m) b.G[m * N + c] = g9[m];
+}
+
+template <int TI, int TJ, int TK>
+__device__ __forceinline__ void stage_tile_split(const BlockDesc& b, int i0, int j0, int k0,
+                                                 const double* __restrict__ qin, const double* q0, double* qout,
+                                                 const Gas& gas, int lim, double dt, int stage) {
+  using T = SplitGeom<TI, TJ, TK>;
+  __shared__ double lds[T::NLDS];
+  double* s_pv = lds;                    // [5][NP]
+  double* s_f = lds + 5 * T::NP;         // [5][NF]
+  const int t = threadIdx.x;
+  const int ni = b.ni, nj = b.nj, nk = b.nk;
+  const int64_t N = b.N;
+  const int64_t si = (int64_t)nj * nk, sj = nk;
+  const double g = gas.gamma;
+  dt = dt * b.dt_scale;
+  const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
+  const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
+  const bool own = gi < ni && gj < nj && gk < nk;
+  const int64_t c = (int64_t)gi * si + (int64_t)gj * sj + gk;
+
+  // ---- 1. primitives of the tile and the two-cell cross halo ------------------
+  for (int h = t; h < T::NP; h += T::NT) {
+    const int3 lc = T::p_cell(h);
+    const int64_t cc =
+        (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
+    double qq[5], pp[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qq[v] = qin[v * N + cc];
+    cons_to_prim_fast(qq, g, pp);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) s_pv[v * T::NP + h] = pp[v];
+  }
+
+  // ---- 2. divergence of the viscous tensor (G of the six wrapped neighbours) ---
+  double kk[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kk[m] = own ? b.kxyz[m * N + c] : 0.0;
+  double vis[4] = {0.0, 0.0, 0.0, 0.0};
+  if (own) {
+    const int64_t cp[3] = {c + (int64_t)(wrap_p(gi, ni) - gi) * si, c + (int64_t)(wrap_p(gj, nj) - gj) * sj,
+                           c + (wrap_p(gk, nk) - gk)};
+    const int64_t cm[3] = {c + (int64_t)(wrap_m(gi, ni) - gi) * si, c + (int64_t)(wrap_m(gj, nj) - gj) * sj,
+                           c + (wrap_m(gk, nk) - gk)};
+    const double* __restrict__ Gb = b.G;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double res = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        const double* Gc = Gb + gidx(r, jj) * N;
+        const double d0 = 0.5 * (Gc[cp[0]] - Gc[cm[0]]);
+        const double d1 = 0.5 * (Gc[cp[1]] - Gc[cm[1]]);
+        const double d2 = 0.5 * (Gc[cp[2]] - Gc[cm[2]]);
+        res = res + (d0 * kk[0 * 3 + jj] + d1 * kk[1 * 3 + jj] + d2 * kk[2 * 3 + jj]);
+      }
+      vis[r] = res;
+    }
+  }
+  __syncthreads();
+
+  // ---- 3. inviscid faces, axis by axis (as the fused kernel) ------------------
+  double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const double rvol = own ? rcp(b.vol[c] + 1e-300) : 0.0;
+#pragma unroll 1
+  for (int d = 0; d < 3; ++d) {
+    const int Td = d == 0 ? TI : (d == 1 ? TJ : TK);
+    const int loc_d = d == 0 ? x : (d == 1 ? y : z);
+    const int gl_d = d == 0 ? gi : (d == 1 ? gj : gk);
+    const int n_d = d == 0 ? ni : (d == 1 ? nj : nk);
+    const int org_d = d == 0 ? i0 : (d == 1 ? j0 : k0);
+    const int64_t st_d = d == 0 ? si : (d == 1 ? sj : (int64_t)1);
+    const int Pd = T::NT / Td;
+    const int o_own = d == 0 ? (y * TK + z) : (d == 1 ? (x * TK + z) : (x * TJ + y));
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+      int f, o;
+      if (pass == 0) { f = loc_d; o = o_own; }
+      else { if (t >= Pd) break; f = Td; o = t; }
+      int lx, ly, lz;
+      if (d == 0) { lx = f; ly = o / TK; lz = o % TK; }
+      else if (d == 1) { lx = o / TK; ly = f; lz = o % TK; }
+      else { lx = o / TJ; ly = o % TJ; lz = f; }
+      const int ai = i0 + lx, aj = j0 + ly, ak = k0 + lz;
+      const int a = org_d + f - 1;          // cell left of the face (unwrapped)
+      if (ai < ni && aj < nj && ak < nk && a >= 1 && a <= n_d - 3) {
+        double p4[4][5];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int slot = T::p_slot(lx + (d == 0 ? s4 - 2 : 0), ly + (d == 1 ? s4 - 2 : 0), lz + (d == 2 ? s4 - 2 : 0));
+#pragma unroll
+          for (int v = 0; v < 5; ++v) p4[s4][v] = s_pv[v * T::NP + slot];
+        }
+        const int64_t cr = (int64_t)ai * si + (int64_t)aj * sj + ak;
+        const int64_t cl = cr - st_d;
+        double kl[3], kr[3];
+        if (pass == 0) {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) kr[m] = d == 0 ? kk[m] : (d == 1 ? kk[3 + m] : kk[6 + m]);
+        } else {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) kr[m] = b.kxyz[(d * 3 + m) * N + cr];
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) kl[m] = b.kxyz[(d * 3 + m) * N + cl];
+        double Fa[5];
+        face_flux_prim(p4[0], p4[1], p4[2], p4[3], kl, kr, g, lim, Fa);
+        const int slot = d == 2 ? o * (TK + 1) + f : f * Pd + o;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) s_f[v * T::NF + slot] = Fa[v];
+      }
+    }
+    __syncthreads();
+    if (own && gl_d >= 2 && gl_d <= n_d - 3) {
+      const int lo = d == 2 ? o_own * (TK + 1) + loc_d : loc_d * Pd + o_own;
+      const int hi = d == 2 ? lo + 1 : lo + Pd;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) inv[v] = inv[v] + (-(s_f[v * T::NF + hi] - s_f[v * T::NF + lo]) * rvol);
+    }
+    __syncthreads();
+  }
+  if (!own) return;
+
+  // ---- 4. rhs and the RK3 combination ----------------------------------------
+  double rhs[5];
+  rhs[0] = inv[0] + 0.0;
+  rhs[1] = inv[1] + vis[0];
+  rhs[2] = inv[2] + vis[1];
+  rhs[3] = inv[3] + vis[2];
+  rhs[4] = inv[4] + vis[3];
+  double qs[5], qq0[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    qs[v] = qin[v * N + c];
+    qq0[v] = stage == 1 ? 0.0 : q0[v * N + c];
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qout[v * N + c] = rk_combine(stage, qq0[v], qs[v], dt, rhs[v]);
+}
+
+template <int TI, int TJ, int TK>
+__global__ void __launch_bounds__(TI * TJ * TK) k_visc_tensor(BlockDesc b, int stage, Gas gas, int ntj, int ntk) {
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
+  const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
+  visc_tile<TI, TJ, TK>(b, ti * TI, tj * TJ, tk * TK, qin, gas);
+}
+
+template <int TI, int TJ, int TK>
+__global__ void __launch_bounds__(TI * TJ * TK) k_stage_split(BlockDesc b, int stage, Gas gas, int lim, double dt,
+                                                              int ntj, int ntk) {
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
+  const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
+  double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
+  stage_tile_split<TI, TJ, TK>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, lim, dt, stage);
+}
+
+template <int TI, int TJ, int TK>
+__global__ void __launch_bounds__(TI * TJ * TK) k_visc_tensor_batched(const BlockDesc* __restrict__ blocks,
+                                                                      const Tile3* __restrict__ tiles, int stage,
+                                                                      Gas gas) {
+  const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+  const BlockDesc& b = blocks[tl.block];
+  const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
+  visc_tile<TI, TJ, TK>(b, tl.i0, tl.j0, tl.k0, qin, gas);
+}
+
+template <int TI, int TJ, int TK>
+__global__ void __launch_bounds__(TI * TJ * TK) k_stage_split_batched(const BlockDesc* __restrict__ blocks,
+                                                                      const Tile3* __restrict__ tiles, int stage,
+                                                                      Gas gas, int lim, double dt) {
+  const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+  const BlockDesc& b = blocks[tl.block];
+  const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
+  double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
+  stage_tile_split<TI, TJ, TK>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, lim, dt, stage);
+}
+
+// Tile shapes the autotuner chooses from (threads = TI*TJ*TK, one cell each):
+// ids 0-6 the fused kernel, ids 7-12 the split pair.
+#define FAMRDP_TILE_VARIANTS(X) \
+  X(0, 4, 4, 16)                \
+  X(1, 4, 8, 16)                \
+  X(2, 8, 4, 16)                \
+  X(3, 2, 8, 16)                \
+  X(4, 4, 4, 32)                \
+  X(5, 8, 8, 8)                 \
+  X(6, 2, 4, 32)
+#define FAMRDP_SPLIT_VARIANTS(X) \
+  X(7, 4, 4, 16)                 \
+  X(8, 4, 8, 8)                  \
+  X(9, 8, 8, 4)                  \
+  X(10, 8, 8, 8)                 \
+  X(11, 2, 8, 16)                \
+  X(12, 4, 8, 16)
+constexpr int NUM_FUSED_VARIANTS = 7;
+constexpr int NUM_TILE_VARIANTS = 13;
+
 inline void tile_dims(int v, int& ti, int& tj, int& tk) {
   switch (v) {
 #define FAMRDP_TD(id, a, b, c) case id: ti = a; tj = b; tk = c; return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TD)
+    FAMRDP_SPLIT_VARIANTS(FAMRDP_TD)
 #undef FAMRDP_TD
     default: ti = 4; tj = 4; tk = 16;
   }
 }
 
-// One stage of one block (plain launch) with tile variant v.
+// One stage of one block (plain launch) with tile variant v (the split pair
+// launches the viscous tensor first, then the stage kernel).
 inline void launch_fused(int v, const BlockDesc& d, int stage, const Gas& gas, int lim, double dt, hipStream_t st) {
   int TI, TJ, TK;
   tile_dims(v, TI, TJ, TK);
@@ -867,10 +1154,18 @@ inline void launch_fused(int v, const BlockDesc& d, int stage, const Gas& gas, i
   case id: k_stage_fused<a, b, c><<<nti * ntj * ntk, a * b * c, 0, st>>>(d, stage, gas, lim, dt, ntj, ntk); return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TL)
 #undef FAMRDP_TL
+#define FAMRDP_SL(id, a, b, c)                                                                          \
+  case id:                                                                                              \
+    k_visc_tensor<a, b, c><<<nti * ntj * ntk, a * b * c, 0, st>>>(d, stage, gas, ntj, ntk);             \
+    k_stage_split<a, b, c><<<nti * ntj * ntk, a * b * c, 0, st>>>(d, stage, gas, lim, dt, ntj, ntk);    \
+    return;
+    FAMRDP_SPLIT_VARIANTS(FAMRDP_SL)
+#undef FAMRDP_SL
   }
 }
 
-// One stage of every block in one launch over a tile table built for v.
+// One stage of every block in one launch (two for the split pair) over a tile
+// table built for v.
 inline void launch_fused_batched(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
                                  const Gas& gas, int lim, double dt, hipStream_t st) {
   switch (v) {
@@ -879,6 +1174,13 @@ inline void launch_fused_batched(int v, const BlockDesc* blocks, const Tile3* ti
     return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TB)
 #undef FAMRDP_TB
+#define FAMRDP_SB(id, a, b, c)                                                                           \
+  case id:                                                                                               \
+    k_visc_tensor_batched<a, b, c><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas);             \
+    k_stage_split_batched<a, b, c><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, lim, dt);    \
+    return;
+    FAMRDP_SPLIT_VARIANTS(FAMRDP_SB)
+#undef FAMRDP_SB
   }
 }
 

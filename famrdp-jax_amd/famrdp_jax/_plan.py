@@ -119,10 +119,11 @@ class Plan:
                                           flags, L.stream()))
 
     def autotune(self, qs, kxyzs, vols, dt, *, batched=False, reps=3):
-        """Time every tile variant of the fused stage kernel on these arrays
-        (stage 1 only, the state is not modified) and keep the fastest.
-        Returns (chosen variant, tile dims, per-variant ms)."""
-        n = 7
+        """Time every stage-kernel variant (tile shapes of the fused kernel and
+        of the split pair) on these arrays (stage 1 only, the state is not
+        modified) and keep the fastest.  Returns (chosen variant, tile dims,
+        per-variant ms)."""
+        n = L.NUM_TILE_VARIANTS
         ms = (L.f64 * n)()
         L.check(self._lib.famrdp_plan_autotune(self._h, self._ptrs(qs), self._ptrs(kxyzs), self._ptrs(vols), float(dt),
                                                1 if batched else 0, int(reps), ms, L.stream()))

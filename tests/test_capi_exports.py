@@ -38,6 +38,12 @@ def test_header_symbols_exported_and_bound():
     assert L.famrdp_last_error() == b""
 
 
+def test_variant_count_matches_header():
+    import re
+    hdr = open(HEADER).read()
+    assert int(re.search(r"#define FAMRDP_NUM_TILE_VARIANTS (\d+)", hdr).group(1)) == _lib.NUM_TILE_VARIANTS
+
+
 def test_float64_guard():
     import numpy as np
     with pytest.raises(TypeError):

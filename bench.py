@@ -72,14 +72,19 @@ def setup_curvilinear(q, xyz_ext, cfg):
     return state, cfg, dt
 
 
-def bench_c2(n, steps, warmup, timing_steps=3):
+def bench_c2(n, steps, warmup, timing_steps=3, tile=-1):
     from famrdp_jax.solver.stepper import Stepper
     state, cfg, dt = setup_c2(n)
     st = Stepper(state, cfg)
     npad = (n + 4) ** 3
-    v, dims, ms = st.autotune(dt)
-    log(f"C2 {n}^3: dt={dt:.4e}; autotuned tile {dims} (variant {v}); stage-1 ms per variant "
-        + " ".join(f"{x:.3f}" for x in ms))
+    if tile >= 0:
+        st.plan.set_tile(tile)
+        v, dims = st.plan.tile()
+        log(f"C2 {n}^3: dt={dt:.4e}; stage-kernel variant {v} (tile {dims}) as requested")
+    else:
+        v, dims, ms = st.autotune(dt)
+        log(f"C2 {n}^3: dt={dt:.4e}; autotuned tile {dims} (variant {v}); stage-1 ms per variant "
+            + " ".join(f"{x:.3f}" for x in ms))
     st.advance(dt, warmup, graph=True)
     _sync()
     t0 = time.perf_counter()
@@ -97,7 +102,7 @@ def bench_c2(n, steps, warmup, timing_steps=3):
     avg_stage_ms = stage_ms / max(nstage, 1)
     bytes_per_stage = npad * BYTES_PER_PADDED_CELL_STEP / 3.0
     achieved = bytes_per_stage / (avg_stage_ms * 1e-3) / 1e9
-    return dict(n=n, npad=npad, dt=dt, steps=steps, wall=wall, ms_per_step=wall * 1e3 / steps, tile=dims,
+    return dict(n=n, npad=npad, dt=dt, steps=steps, wall=wall, ms_per_step=wall * 1e3 / steps, tile=dims, variant=v,
                 value=n ** 3 * steps / wall, avg_stage_ms=avg_stage_ms, stage_bytes=bytes_per_stage,
                 achieved_gbs=achieved, finite=finite)
 
@@ -209,6 +214,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--tile", type=int, default=-1, help="stage-kernel variant (default: autotune)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 2 GiB float64 buffer 5 times (counter calibration)")
     args = ap.parse_args()
@@ -230,7 +236,7 @@ def main():
         return
     from famrdp_jax import _lib
     _lib.load()
-    c2 = bench_c2(args.n, args.steps, args.warmup)
+    c2 = bench_c2(args.n, args.steps, args.warmup, tile=args.tile)
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:

@@ -583,12 +583,34 @@ struct FusedGeom {
   static constexpr int F0 = (TI + 1) * TJ * TK, F1 = TI * (TJ + 1) * TK, F2 = TI * TJ * (TK + 1);
   static constexpr int NF = F0 > F1 ? (F0 > F2 ? F0 : F2) : (F1 > F2 ? F1 : F2);
   static constexpr int NLDS = 5 * NPV + (9 * NG > 5 * NF ? 9 * NG : 5 * NF);
+  static constexpr int NPVS = NPV;   // primitive slots (stride of a variable)
 
+  // slot of the cell (lx, ly, lz) whose coordinate along D lies in
+  // [-2, T_D + 1] and whose other two coordinates lie in the tile
+  template <int D>
+  __device__ static __forceinline__ int axis_slot(int lx, int ly, int lz) {
+    if constexpr (D == 0) {
+      const bool out = (unsigned)(lx + 1) > (unsigned)(TI + 1);
+      return out ? NR0 + ((lx < 0 ? 0 : 1) * TJ + ly) * TK + lz : ((lx + 1) * B0J + (ly + 1)) * B0K + (lz + 1);
+    } else if constexpr (D == 1) {
+      const bool out = (unsigned)(ly + 1) > (unsigned)(TJ + 1);
+      return out ? NR0 + NR1 + ((ly < 0 ? 0 : 1) * TI + lx) * TK + lz : ((lx + 1) * B0J + (ly + 1)) * B0K + (lz + 1);
+    } else {
+      const bool out = (unsigned)(lz + 1) > (unsigned)(TK + 1);
+      return out ? NR0 + NR1 + NR2 + ((lz < 0 ? 0 : 1) * TI + lx) * TJ + ly
+                 : ((lx + 1) * B0J + (ly + 1)) * B0K + (lz + 1);
+    }
+  }
+
+  // (written with selects only: no divergent branches in the face loops)
   __device__ static __forceinline__ int pv_slot(int x, int y, int z) {
-    if (x >= -1 && x <= TI && y >= -1 && y <= TJ && z >= -1 && z <= TK) return ((x + 1) * B0J + (y + 1)) * B0K + (z + 1);
-    if (x < -1 || x > TI) return NR0 + ((x < 0 ? 0 : 1) * TJ + y) * TK + z;
-    if (y < -1 || y > TJ) return NR0 + NR1 + ((y < 0 ? 0 : 1) * TI + x) * TK + z;
-    return NR0 + NR1 + NR2 + ((z < 0 ? 0 : 1) * TI + x) * TJ + y;
+    const bool ox = (unsigned)(x + 1) > (unsigned)(TI + 1), oy = (unsigned)(y + 1) > (unsigned)(TJ + 1);
+    const bool oz = (unsigned)(z + 1) > (unsigned)(TK + 1);
+    const int s_box = ((x + 1) * B0J + (y + 1)) * B0K + (z + 1);
+    const int s_x = NR0 + ((x < 0 ? 0 : 1) * TJ + y) * TK + z;
+    const int s_y = NR0 + NR1 + ((y < 0 ? 0 : 1) * TI + x) * TK + z;
+    const int s_z = NR0 + NR1 + NR2 + ((z < 0 ? 0 : 1) * TI + x) * TJ + y;
+    return ox ? s_x : (oy ? s_y : (oz ? s_z : s_box));
   }
   // local coordinates of LDS primitive slot h (inverse of pv_slot)
   __device__ static __forceinline__ int3 pv_cell(int h) {
@@ -601,10 +623,12 @@ struct FusedGeom {
     return make_int3((h / TJ) % TI, h % TJ, h / (TI * TJ) ? TK + 1 : -2);
   }
   __device__ static __forceinline__ int g_slot(int x, int y, int z) {
-    if (x >= 0 && x < TI && y >= 0 && y < TJ && z >= 0 && z < TK) return (x * TJ + y) * TK + z;
-    if (x < 0 || x >= TI) return NT + ((x < 0 ? 0 : 1) * TJ + y) * TK + z;
-    if (y < 0 || y >= TJ) return NT + 2 * TJ * TK + ((y < 0 ? 0 : 1) * TI + x) * TK + z;
-    return NT + 2 * TJ * TK + 2 * TI * TK + ((z < 0 ? 0 : 1) * TI + x) * TJ + y;
+    const bool ox = (unsigned)x >= (unsigned)TI, oy = (unsigned)y >= (unsigned)TJ, oz = (unsigned)z >= (unsigned)TK;
+    const int s_in = (x * TJ + y) * TK + z;
+    const int s_x = NT + ((x < 0 ? 0 : 1) * TJ + y) * TK + z;
+    const int s_y = NT + 2 * TJ * TK + ((y < 0 ? 0 : 1) * TI + x) * TK + z;
+    const int s_z = NT + 2 * TJ * TK + 2 * TI * TK + ((z < 0 ? 0 : 1) * TI + x) * TJ + y;
+    return ox ? s_x : (oy ? s_y : (oz ? s_z : s_in));
   }
   // local coordinates of G slot h (inverse of g_slot)
   __device__ static __forceinline__ int3 g_cell(int h) {
@@ -634,10 +658,81 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
   return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-template <int TI, int TJ, int TK>
+// ---------------------------------------------------------------------------
+// Inviscid faces of one axis D for a tile (rhs/invscd.py:53-131), shared by
+// the fused and split stage kernels.  Faces of the tile: the low face of
+// every cell (computed by the cell's own thread, with kxyz of the cell from
+// registers) and the layer of high faces at the far end of the tile (dealt
+// to the threads beyond the tile's cells, or after the low face when there
+// are none).  Face fluxes go to the LDS buffer s_f (axes i, j: f * Pd + o,
+// axis k: o * (TK + 1) + f, face f between local cells f-1 and f); after a
+// barrier every cell with 2 <= c <= n-3 along D takes -(F(c+1/2) -
+// F(c-1/2)) / vol.  Ends with a barrier (s_f is reused by the next axis).
+// ---------------------------------------------------------------------------
+template <int D, class T, int TI, int TJ, int TK, int NTH, int LIM>
+__device__ __forceinline__ void face_axis(const BlockDesc& b, int i0, int j0, int k0, const double* s_pv,
+                                          double* s_f, const double kk[9], bool own, int t, int x, int y, int z,
+                                          double rvol, double g, double inv[5]) {
+  constexpr int NT = TI * TJ * TK;
+  constexpr int Td = D == 0 ? TI : (D == 1 ? TJ : TK);
+  constexpr int Pd = NT / Td;
+  const int ni = b.ni, nj = b.nj, nk = b.nk;
+  const int64_t N = b.N;
+  const int64_t si = (int64_t)nj * nk, sj = nk;
+  const int n_d = D == 0 ? ni : (D == 1 ? nj : nk);
+  const int org_d = D == 0 ? i0 : (D == 1 ? j0 : k0);
+  constexpr int64_t one = 1;
+  const int64_t st_d = D == 0 ? si : (D == 1 ? sj : one);
+  auto face = [&](int f, int lx, int ly, int lz, int slot, bool kr_own) {
+    const int ai = i0 + lx, aj = j0 + ly, ak = k0 + lz;
+    const int a = org_d + f - 1;          // cell left of the face (unwrapped)
+    if (!(ai < ni && aj < nj && ak < nk && a >= 1 && a <= n_d - 3)) return;
+    double p4[4][5];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+      const int sl = T::template axis_slot<D>(lx + (D == 0 ? s4 - 2 : 0), ly + (D == 1 ? s4 - 2 : 0),
+                                              lz + (D == 2 ? s4 - 2 : 0));
+#pragma unroll
+      for (int v = 0; v < 5; ++v) p4[s4][v] = s_pv[v * T::NPVS + sl];
+    }
+    const int64_t cr = (int64_t)ai * si + (int64_t)aj * sj + ak;
+    const int64_t cl = cr - st_d;
+    double kl[3], kr[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      kl[m] = b.kxyz[(D * 3 + m) * N + cl];
+      kr[m] = kr_own ? kk[D * 3 + m] : b.kxyz[(D * 3 + m) * N + cr];
+    }
+    double Fa[5];
+    face_flux_prim<LIM>(p4[0], p4[1], p4[2], p4[3], kl, kr, g, Fa);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) s_f[v * T::NF + slot] = Fa[v];
+  };
+  const int loc_d = D == 0 ? x : (D == 1 ? y : z);
+  const int o_own = D == 0 ? (y * TK + z) : (D == 1 ? (x * TK + z) : (x * TJ + y));
+  if (t < NT) face(loc_d, x, y, z, D == 2 ? o_own * (TK + 1) + loc_d : loc_d * Pd + o_own, true);
+  for (int h = (t >= NT ? t - NT : t + NTH - NT); h < Pd; h += NTH) {
+    int lx, ly, lz;
+    if constexpr (D == 0) { lx = TI; ly = h / TK; lz = h % TK; }
+    else if constexpr (D == 1) { lx = h / TK; ly = TJ; lz = h % TK; }
+    else { lx = h / TJ; ly = h % TJ; lz = TK; }
+    face(Td, lx, ly, lz, D == 2 ? h * (TK + 1) + Td : Td * Pd + h, false);
+  }
+  __syncthreads();
+  const int gl_d = D == 0 ? i0 + x : (D == 1 ? j0 + y : k0 + z);
+  if (own && gl_d >= 2 && gl_d <= n_d - 3) {
+    const int lo = D == 2 ? o_own * (TK + 1) + loc_d : loc_d * Pd + o_own;
+    const int hi = D == 2 ? lo + 1 : lo + Pd;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) inv[v] = inv[v] + (-(s_f[v * T::NF + hi] - s_f[v * T::NF + lo]) * rvol);
+  }
+  __syncthreads();
+}
+
+template <int TI, int TJ, int TK, int LIM>
 __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int j0, int k0,
                                                  const double* __restrict__ qin, const double* q0, double* qout,
-                                                 const Gas& gas, int lim, double dt, int stage) {
+                                                 const Gas& gas, double dt, int stage) {
   using T = FusedGeom<TI, TJ, TK>;
   __shared__ double lds[T::NLDS];
   double* s_pv = lds;                    // [5][NPV]
@@ -740,63 +835,9 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
   // ---- 3. inviscid faces, axis by axis ---------------------------------------
   double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const double rvol = own ? rcp(b.vol[c] + 1e-300) : 0.0;
-#pragma unroll 1
-  for (int d = 0; d < 3; ++d) {
-    const int Td = d == 0 ? TI : (d == 1 ? TJ : TK);
-    const int loc_d = d == 0 ? x : (d == 1 ? y : z);
-    const int gl_d = d == 0 ? gi : (d == 1 ? gj : gk);
-    const int n_d = d == 0 ? ni : (d == 1 ? nj : nk);
-    const int org_d = d == 0 ? i0 : (d == 1 ? j0 : k0);
-    const int64_t st_d = d == 0 ? si : (d == 1 ? sj : (int64_t)1);
-    const int Pd = T::NT / Td;
-    const int o_own = d == 0 ? (y * TK + z) : (d == 1 ? (x * TK + z) : (x * TJ + y));
-#pragma unroll 1
-    for (int pass = 0; pass < 2; ++pass) {
-      int f, o;
-      if (pass == 0) { f = loc_d; o = o_own; }
-      else { if (t >= Pd) break; f = Td; o = t; }
-      int lx, ly, lz;
-      if (d == 0) { lx = f; ly = o / TK; lz = o % TK; }
-      else if (d == 1) { lx = o / TK; ly = f; lz = o % TK; }
-      else { lx = o / TJ; ly = o % TJ; lz = f; }
-      const int ai = i0 + lx, aj = j0 + ly, ak = k0 + lz;
-      const int a = org_d + f - 1;          // cell left of the face (unwrapped)
-      if (ai < ni && aj < nj && ak < nk && a >= 1 && a <= n_d - 3) {
-        double p4[4][5];
-#pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-          const int slot = T::pv_slot(lx + (d == 0 ? s4 - 2 : 0), ly + (d == 1 ? s4 - 2 : 0), lz + (d == 2 ? s4 - 2 : 0));
-#pragma unroll
-          for (int v = 0; v < 5; ++v) p4[s4][v] = s_pv[v * T::NPV + slot];
-        }
-        const int64_t cr = (int64_t)ai * si + (int64_t)aj * sj + ak;
-        const int64_t cl = cr - st_d;
-        double kl[3], kr[3];
-        if (pass == 0) {
-#pragma unroll
-          for (int m = 0; m < 3; ++m) kr[m] = d == 0 ? kk[m] : (d == 1 ? kk[3 + m] : kk[6 + m]);
-        } else {
-#pragma unroll
-          for (int m = 0; m < 3; ++m) kr[m] = b.kxyz[(d * 3 + m) * N + cr];
-        }
-#pragma unroll
-        for (int m = 0; m < 3; ++m) kl[m] = b.kxyz[(d * 3 + m) * N + cl];
-        double Fa[5];
-        face_flux_prim(p4[0], p4[1], p4[2], p4[3], kl, kr, g, lim, Fa);
-        const int slot = d == 2 ? o * (TK + 1) + f : f * Pd + o;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) s_f[v * T::NF + slot] = Fa[v];
-      }
-    }
-    __syncthreads();
-    if (own && gl_d >= 2 && gl_d <= n_d - 3) {
-      const int lo = d == 2 ? o_own * (TK + 1) + loc_d : loc_d * Pd + o_own;
-      const int hi = d == 2 ? lo + 1 : lo + Pd;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) inv[v] = inv[v] + (-(s_f[v * T::NF + hi] - s_f[v * T::NF + lo]) * rvol);
-    }
-    __syncthreads();
-  }
+  face_axis<0, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
+  face_axis<1, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
+  face_axis<2, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
   if (!own) return;
 
   // ---- 4. rhs and the RK3 combination ----------------------------------------
@@ -816,25 +857,25 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
   for (int v = 0; v < 5; ++v) qout[v * N + c] = rk_combine(stage, qq0[v], qs[v], dt, rhs[v]);
 }
 
-template <int TI, int TJ, int TK>
-__global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused(BlockDesc b, int stage, Gas gas, int lim, double dt,
-                                                              int ntj, int ntk) {
+template <int TI, int TJ, int TK, int LIM>
+__global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused(BlockDesc b, int stage, Gas gas, double dt, int ntj,
+                                                              int ntk) {
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-  stage_tile_fused<TI, TJ, TK>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, lim, dt, stage);
+  stage_tile_fused<TI, TJ, TK, LIM>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, dt, stage);
 }
 
-template <int TI, int TJ, int TK>
+template <int TI, int TJ, int TK, int LIM>
 __global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused_batched(const BlockDesc* __restrict__ blocks,
                                                                       const Tile3* __restrict__ tiles, int stage,
-                                                                      Gas gas, int lim, double dt) {
+                                                                      Gas gas, double dt) {
   const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
   const BlockDesc& b = blocks[tl.block];
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-  stage_tile_fused<TI, TJ, TK>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, lim, dt, stage);
+  stage_tile_fused<TI, TJ, TK, LIM>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, dt, stage);
 }
 
 // ===========================================================================
@@ -860,13 +901,37 @@ struct SplitGeom {
   static constexpr int F0 = (TI + 1) * TJ * TK, F1 = TI * (TJ + 1) * TK, F2 = TI * TJ * (TK + 1);
   static constexpr int NF = F0 > F1 ? (F0 > F2 ? F0 : F2) : (F1 > F2 ? F1 : F2);
   static constexpr int NLDS = 5 * NP + 5 * NF;
+  static constexpr int NPVS = NP;    // primitive slots (stride of a variable)
+
+  // slot of the cell (lx, ly, lz) whose coordinate along D lies in
+  // [-2, T_D + 1] and whose other two coordinates lie in the tile
+  template <int D>
+  __device__ static __forceinline__ int axis_slot(int lx, int ly, int lz) {
+    if constexpr (D == 0) {
+      const bool out = (unsigned)lx >= (unsigned)TI;
+      return out ? NT + ((lx < 0 ? lx + 2 : lx - TI + 2) * TJ + ly) * TK + lz : (lx * TJ + ly) * TK + lz;
+    } else if constexpr (D == 1) {
+      const bool out = (unsigned)ly >= (unsigned)TJ;
+      return out ? NT + 4 * A0 + ((ly < 0 ? ly + 2 : ly - TJ + 2) * TI + lx) * TK + lz : (lx * TJ + ly) * TK + lz;
+    } else {
+      const bool out = (unsigned)lz >= (unsigned)TK;
+      return out ? NT + 4 * (A0 + A1) + ((lz < 0 ? lz + 2 : lz - TK + 2) * TI + lx) * TJ + ly
+                 : (lx * TJ + ly) * TK + lz;
+    }
+  }
+  // threads that give every axis's faces one pass (a multiple of 64)
+  static constexpr int NTH1 = (NF + 63) / 64 * 64;
 
   // slot of a cell that lies in the tile or on the halo layers of one axis
+  // (written with selects only: no divergent branches in the face loops)
   __device__ static __forceinline__ int p_slot(int x, int y, int z) {
-    if (x < 0 || x >= TI) return NT + ((x < 0 ? x + 2 : x - TI + 2) * TJ + y) * TK + z;
-    if (y < 0 || y >= TJ) return NT + 4 * A0 + ((y < 0 ? y + 2 : y - TJ + 2) * TI + x) * TK + z;
-    if (z < 0 || z >= TK) return NT + 4 * (A0 + A1) + ((z < 0 ? z + 2 : z - TK + 2) * TI + x) * TJ + y;
-    return (x * TJ + y) * TK + z;
+    const bool ox = (unsigned)x >= (unsigned)TI, oy = (unsigned)y >= (unsigned)TJ, oz = (unsigned)z >= (unsigned)TK;
+    const int lx = x < 0 ? x + 2 : x - TI + 2, ly = y < 0 ? y + 2 : y - TJ + 2, lz = z < 0 ? z + 2 : z - TK + 2;
+    const int s_in = (x * TJ + y) * TK + z;
+    const int s_x = NT + (lx * TJ + y) * TK + z;
+    const int s_y = NT + 4 * A0 + (ly * TI + x) * TK + z;
+    const int s_z = NT + 4 * (A0 + A1) + (lz * TI + x) * TJ + y;
+    return ox ? s_x : (oy ? s_y : (oz ? s_z : s_in));
   }
   // inverse of p_slot
   __device__ static __forceinline__ int3 p_cell(int h) {
@@ -936,11 +1001,12 @@ __device__ __forceinline__ void visc_tile(const BlockDesc& b, int i0, int j0, in
   for (int m = 0; m < 9; ++m) b.G[m * N + c] = g9[m];
 }
 
-template <int TI, int TJ, int TK>
+template <int TI, int TJ, int TK, int NTH, int LIM>
 __device__ __forceinline__ void stage_tile_split(const BlockDesc& b, int i0, int j0, int k0,
                                                  const double* __restrict__ qin, const double* q0, double* qout,
-                                                 const Gas& gas, int lim, double dt, int stage) {
+                                                 const Gas& gas, double dt, int stage) {
   using T = SplitGeom<TI, TJ, TK>;
+  static_assert(NTH >= T::NT && NTH % 64 == 0, "stage kernel needs a thread per cell");
   __shared__ double lds[T::NLDS];
   double* s_pv = lds;                    // [5][NP]
   double* s_f = lds + 5 * T::NP;         // [5][NF]
@@ -950,13 +1016,14 @@ __device__ __forceinline__ void stage_tile_split(const BlockDesc& b, int i0, int
   const int64_t si = (int64_t)nj * nk, sj = nk;
   const double g = gas.gamma;
   dt = dt * b.dt_scale;
+  // threads t < NT own cell (x, y, z) of the tile; all NTH threads compute faces
   const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
   const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
-  const bool own = gi < ni && gj < nj && gk < nk;
+  const bool own = t < T::NT && gi < ni && gj < nj && gk < nk;
   const int64_t c = (int64_t)gi * si + (int64_t)gj * sj + gk;
 
   // ---- 1. primitives of the tile and the two-cell cross halo ------------------
-  for (int h = t; h < T::NP; h += T::NT) {
+  for (int h = t; h < T::NP; h += NTH) {
     const int3 lc = T::p_cell(h);
     const int64_t cc =
         (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
@@ -995,66 +1062,12 @@ __device__ __forceinline__ void stage_tile_split(const BlockDesc& b, int i0, int
   }
   __syncthreads();
 
-  // ---- 3. inviscid faces, axis by axis (as the fused kernel) ------------------
+  // ---- 3. inviscid faces, axis by axis ---------------------------------------
   double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const double rvol = own ? rcp(b.vol[c] + 1e-300) : 0.0;
-#pragma unroll 1
-  for (int d = 0; d < 3; ++d) {
-    const int Td = d == 0 ? TI : (d == 1 ? TJ : TK);
-    const int loc_d = d == 0 ? x : (d == 1 ? y : z);
-    const int gl_d = d == 0 ? gi : (d == 1 ? gj : gk);
-    const int n_d = d == 0 ? ni : (d == 1 ? nj : nk);
-    const int org_d = d == 0 ? i0 : (d == 1 ? j0 : k0);
-    const int64_t st_d = d == 0 ? si : (d == 1 ? sj : (int64_t)1);
-    const int Pd = T::NT / Td;
-    const int o_own = d == 0 ? (y * TK + z) : (d == 1 ? (x * TK + z) : (x * TJ + y));
-#pragma unroll 1
-    for (int pass = 0; pass < 2; ++pass) {
-      int f, o;
-      if (pass == 0) { f = loc_d; o = o_own; }
-      else { if (t >= Pd) break; f = Td; o = t; }
-      int lx, ly, lz;
-      if (d == 0) { lx = f; ly = o / TK; lz = o % TK; }
-      else if (d == 1) { lx = o / TK; ly = f; lz = o % TK; }
-      else { lx = o / TJ; ly = o % TJ; lz = f; }
-      const int ai = i0 + lx, aj = j0 + ly, ak = k0 + lz;
-      const int a = org_d + f - 1;          // cell left of the face (unwrapped)
-      if (ai < ni && aj < nj && ak < nk && a >= 1 && a <= n_d - 3) {
-        double p4[4][5];
-#pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-          const int slot = T::p_slot(lx + (d == 0 ? s4 - 2 : 0), ly + (d == 1 ? s4 - 2 : 0), lz + (d == 2 ? s4 - 2 : 0));
-#pragma unroll
-          for (int v = 0; v < 5; ++v) p4[s4][v] = s_pv[v * T::NP + slot];
-        }
-        const int64_t cr = (int64_t)ai * si + (int64_t)aj * sj + ak;
-        const int64_t cl = cr - st_d;
-        double kl[3], kr[3];
-        if (pass == 0) {
-#pragma unroll
-          for (int m = 0; m < 3; ++m) kr[m] = d == 0 ? kk[m] : (d == 1 ? kk[3 + m] : kk[6 + m]);
-        } else {
-#pragma unroll
-          for (int m = 0; m < 3; ++m) kr[m] = b.kxyz[(d * 3 + m) * N + cr];
-        }
-#pragma unroll
-        for (int m = 0; m < 3; ++m) kl[m] = b.kxyz[(d * 3 + m) * N + cl];
-        double Fa[5];
-        face_flux_prim(p4[0], p4[1], p4[2], p4[3], kl, kr, g, lim, Fa);
-        const int slot = d == 2 ? o * (TK + 1) + f : f * Pd + o;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) s_f[v * T::NF + slot] = Fa[v];
-      }
-    }
-    __syncthreads();
-    if (own && gl_d >= 2 && gl_d <= n_d - 3) {
-      const int lo = d == 2 ? o_own * (TK + 1) + loc_d : loc_d * Pd + o_own;
-      const int hi = d == 2 ? lo + 1 : lo + Pd;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) inv[v] = inv[v] + (-(s_f[v * T::NF + hi] - s_f[v * T::NF + lo]) * rvol);
-    }
-    __syncthreads();
-  }
+  face_axis<0, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
+  face_axis<1, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
+  face_axis<2, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
   if (!own) return;
 
   // ---- 4. rhs and the RK3 combination ----------------------------------------
@@ -1082,14 +1095,13 @@ __global__ void __launch_bounds__(TI * TJ * TK) k_visc_tensor(BlockDesc b, int s
   visc_tile<TI, TJ, TK>(b, ti * TI, tj * TJ, tk * TK, qin, gas);
 }
 
-template <int TI, int TJ, int TK>
-__global__ void __launch_bounds__(TI * TJ * TK) k_stage_split(BlockDesc b, int stage, Gas gas, int lim, double dt,
-                                                              int ntj, int ntk) {
+template <int TI, int TJ, int TK, int NTH, int LIM>
+__global__ void __launch_bounds__(NTH) k_stage_split(BlockDesc b, int stage, Gas gas, double dt, int ntj, int ntk) {
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-  stage_tile_split<TI, TJ, TK>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, lim, dt, stage);
+  stage_tile_split<TI, TJ, TK, NTH, LIM>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, dt, stage);
 }
 
 template <int TI, int TJ, int TK>
@@ -1102,86 +1114,100 @@ __global__ void __launch_bounds__(TI * TJ * TK) k_visc_tensor_batched(const Bloc
   visc_tile<TI, TJ, TK>(b, tl.i0, tl.j0, tl.k0, qin, gas);
 }
 
-template <int TI, int TJ, int TK>
-__global__ void __launch_bounds__(TI * TJ * TK) k_stage_split_batched(const BlockDesc* __restrict__ blocks,
-                                                                      const Tile3* __restrict__ tiles, int stage,
-                                                                      Gas gas, int lim, double dt) {
+template <int TI, int TJ, int TK, int NTH, int LIM>
+__global__ void __launch_bounds__(NTH) k_stage_split_batched(const BlockDesc* __restrict__ blocks,
+                                                             const Tile3* __restrict__ tiles, int stage, Gas gas,
+                                                             double dt) {
   const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
   const BlockDesc& b = blocks[tl.block];
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-  stage_tile_split<TI, TJ, TK>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, lim, dt, stage);
+  stage_tile_split<TI, TJ, TK, NTH, LIM>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, dt, stage);
 }
 
-// Tile shapes the autotuner chooses from (threads = TI*TJ*TK, one cell each):
-// ids 0-6 the fused kernel, ids 7-12 the split pair.
+// Stage-kernel variants the autotuner chooses from.  Fused kernel:
+// (id, TI, TJ, TK), one thread per cell.  Split pair: (id, TI, TJ, TK,
+// threads of the stage kernel): with one thread per cell an axis's faces
+// take two passes, with a thread per face (NTH1) one.
 #define FAMRDP_TILE_VARIANTS(X) \
   X(0, 4, 4, 16)                \
   X(1, 4, 8, 16)                \
   X(2, 8, 4, 16)                \
-  X(3, 2, 8, 16)                \
-  X(4, 4, 4, 32)                \
-  X(5, 8, 8, 8)                 \
-  X(6, 2, 4, 32)
-#define FAMRDP_SPLIT_VARIANTS(X) \
-  X(7, 4, 4, 16)                 \
-  X(8, 4, 8, 8)                  \
-  X(9, 8, 8, 4)                  \
-  X(10, 8, 8, 8)                 \
-  X(11, 2, 8, 16)                \
-  X(12, 4, 8, 16)
-constexpr int NUM_FUSED_VARIANTS = 7;
-constexpr int NUM_TILE_VARIANTS = 13;
+  X(3, 4, 4, 32)                \
+  X(4, 8, 8, 8)
+#define FAMRDP_SPLIT_VARIANTS(X)  \
+  X(5, 4, 4, 16, 256)             \
+  X(6, 4, 8, 8, 256)              \
+  X(7, 8, 8, 8, 512)              \
+  X(8, 2, 8, 16, 256)             \
+  X(9, 4, 8, 8, 320)
+constexpr int NUM_FUSED_VARIANTS = 5;
+constexpr int NUM_TILE_VARIANTS = 10;
 
 inline void tile_dims(int v, int& ti, int& tj, int& tk) {
   switch (v) {
 #define FAMRDP_TD(id, a, b, c) case id: ti = a; tj = b; tk = c; return;
+#define FAMRDP_TD5(id, a, b, c, n) case id: ti = a; tj = b; tk = c; return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TD)
-    FAMRDP_SPLIT_VARIANTS(FAMRDP_TD)
+    FAMRDP_SPLIT_VARIANTS(FAMRDP_TD5)
+#undef FAMRDP_TD5
 #undef FAMRDP_TD
     default: ti = 4; tj = 4; tk = 16;
   }
 }
 
-// One stage of one block (plain launch) with tile variant v (the split pair
-// launches the viscous tensor first, then the stage kernel).
-inline void launch_fused(int v, const BlockDesc& d, int stage, const Gas& gas, int lim, double dt, hipStream_t st) {
+// One stage of one block (plain launch) with variant v; the limiter is a
+// template parameter of the stage kernels (LIM_MINMOD / LIM_VANLEER).
+template <int LIM>
+inline void launch_stage_lim(int v, const BlockDesc& d, int stage, const Gas& gas, double dt, hipStream_t st) {
   int TI, TJ, TK;
   tile_dims(v, TI, TJ, TK);
   const int nti = (d.ni + TI - 1) / TI, ntj = (d.nj + TJ - 1) / TJ, ntk = (d.nk + TK - 1) / TK;
+  const int nt = nti * ntj * ntk;
   switch (v) {
 #define FAMRDP_TL(id, a, b, c) \
-  case id: k_stage_fused<a, b, c><<<nti * ntj * ntk, a * b * c, 0, st>>>(d, stage, gas, lim, dt, ntj, ntk); return;
+  case id: k_stage_fused<a, b, c, LIM><<<nt, a * b * c, 0, st>>>(d, stage, gas, dt, ntj, ntk); return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TL)
 #undef FAMRDP_TL
-#define FAMRDP_SL(id, a, b, c)                                                                          \
-  case id:                                                                                              \
-    k_visc_tensor<a, b, c><<<nti * ntj * ntk, a * b * c, 0, st>>>(d, stage, gas, ntj, ntk);             \
-    k_stage_split<a, b, c><<<nti * ntj * ntk, a * b * c, 0, st>>>(d, stage, gas, lim, dt, ntj, ntk);    \
+#define FAMRDP_SL(id, a, b, c, n)                                                       \
+  case id:                                                                              \
+    k_visc_tensor<a, b, c><<<nt, a * b * c, 0, st>>>(d, stage, gas, ntj, ntk);          \
+    k_stage_split<a, b, c, n, LIM><<<nt, n, 0, st>>>(d, stage, gas, dt, ntj, ntk);      \
     return;
     FAMRDP_SPLIT_VARIANTS(FAMRDP_SL)
 #undef FAMRDP_SL
   }
 }
 
+inline void launch_fused(int v, const BlockDesc& d, int stage, const Gas& gas, int lim, double dt, hipStream_t st) {
+  if (lim == LIM_VANLEER) launch_stage_lim<LIM_VANLEER>(v, d, stage, gas, dt, st);
+  else launch_stage_lim<LIM_MINMOD>(v, d, stage, gas, dt, st);
+}
+
 // One stage of every block in one launch (two for the split pair) over a tile
 // table built for v.
-inline void launch_fused_batched(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
-                                 const Gas& gas, int lim, double dt, hipStream_t st) {
+template <int LIM>
+inline void launch_stage_batched_lim(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
+                                     const Gas& gas, double dt, hipStream_t st) {
   switch (v) {
-#define FAMRDP_TB(id, a, b, c)                                                                          \
-  case id: k_stage_fused_batched<a, b, c><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, lim, dt); \
-    return;
+#define FAMRDP_TB(id, a, b, c) \
+  case id: k_stage_fused_batched<a, b, c, LIM><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, dt); return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TB)
 #undef FAMRDP_TB
-#define FAMRDP_SB(id, a, b, c)                                                                           \
-  case id:                                                                                               \
-    k_visc_tensor_batched<a, b, c><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas);             \
-    k_stage_split_batched<a, b, c><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, lim, dt);    \
+#define FAMRDP_SB(id, a, b, c, n)                                                                  \
+  case id:                                                                                         \
+    k_visc_tensor_batched<a, b, c><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas);       \
+    k_stage_split_batched<a, b, c, n, LIM><<<ntiles, n, 0, st>>>(blocks, tiles, stage, gas, dt);   \
     return;
     FAMRDP_SPLIT_VARIANTS(FAMRDP_SB)
 #undef FAMRDP_SB
   }
+}
+
+inline void launch_fused_batched(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
+                                 const Gas& gas, int lim, double dt, hipStream_t st) {
+  if (lim == LIM_VANLEER) launch_stage_batched_lim<LIM_VANLEER>(v, blocks, tiles, ntiles, stage, gas, dt, st);
+  else launch_stage_batched_lim<LIM_MINMOD>(v, blocks, tiles, ntiles, stage, gas, dt, st);
 }
 
 }  // namespace famrdp

@@ -38,9 +38,9 @@ class Bc(ctypes.Structure):
 
 # (name, argtypes) of every symbol in include/famrdp_capi.h
 # stage-kernel variants of famrdp_plan_autotune / famrdp_plan_set_tile:
-# 0-6 tile shapes of the fused kernel, 7-12 of the split pair
-NUM_TILE_VARIANTS = 13
-NUM_FUSED_VARIANTS = 7
+# 0-4 tile shapes of the fused kernel, 5-9 of the split pair
+NUM_TILE_VARIANTS = 10
+NUM_FUSED_VARIANTS = 5
 
 SIGNATURES = {
     "famrdp_last_error": ([], ctypes.c_char_p),

@@ -67,6 +67,7 @@ struct famrdp_plan {
   int ntiles3[NUM_TILE_VARIANTS] = {};
   int tile_variant = 0;
   int persistent_grid = 0;
+  int ncu = 256;                  // compute units of the device the plan was made on
   // workspace
   std::vector<double*> q1, q2, G;
   // graph cache
@@ -234,11 +235,11 @@ int launch_stages(famrdp_plan* p, double dt, int flags, hipStream_t st, bool adj
     if (tm) HIP_TRY(hipEventRecord(e0, st));
     if (!direct && (flags & FAMRDP_STEP_BATCHED)) {
       const int v = p->tile_variant;
-      launch_fused_batched(v, dtab, p->d_tiles3[v], p->ntiles3[v], stage, p->gas, p->limiter, dt, st);
+      launch_fused_batched(v, dtab, p->d_tiles3[v], p->ntiles3[v], stage, p->gas, p->limiter, dt, p->ncu, st);
       LAUNCH_CHECK();
     } else if (!direct) {
       for (int b = 0; b < p->nblocks; ++b) {
-        launch_fused(p->tile_variant, htab[b], stage, p->gas, p->limiter, dt, st);
+        launch_fused(p->tile_variant, htab[b], stage, p->gas, p->limiter, dt, p->ncu, st);
         LAUNCH_CHECK();
       }
     } else if (flags & FAMRDP_STEP_BATCHED) {
@@ -632,6 +633,7 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   }
   p->persistent_grid = std::min(p->ntiles, ncu * 8);
+  p->ncu = ncu;
   *out = p;
   return FAMRDP_OK;
 }
@@ -668,9 +670,9 @@ int famrdp_plan_autotune(famrdp_plan* p, double* const* q, const double* const* 
     for (int r = 0; r <= reps; ++r) {
       if (r == 1) HIP_TRY(hipEventRecord(e0, st));
       if (flags & FAMRDP_STEP_BATCHED) {
-        launch_fused_batched(v, p->d_blocks, p->d_tiles3[v], p->ntiles3[v], 1, p->gas, p->limiter, dt, st);
+        launch_fused_batched(v, p->d_blocks, p->d_tiles3[v], p->ntiles3[v], 1, p->gas, p->limiter, dt, p->ncu, st);
       } else {
-        for (int b = 0; b < p->nblocks; ++b) launch_fused(v, p->h_blocks[b], 1, p->gas, p->limiter, dt, st);
+        for (int b = 0; b < p->nblocks; ++b) launch_fused(v, p->h_blocks[b], 1, p->gas, p->limiter, dt, p->ncu, st);
       }
       LAUNCH_CHECK();
     }

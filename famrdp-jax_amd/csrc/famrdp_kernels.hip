@@ -582,7 +582,8 @@ struct FusedGeom {
   static constexpr int NG = NT + 2 * (TJ * TK + TI * TK + TI * TJ);   // tile + cross halo 1
   static constexpr int F0 = (TI + 1) * TJ * TK, F1 = TI * (TJ + 1) * TK, F2 = TI * TJ * (TK + 1);
   static constexpr int NF = F0 > F1 ? (F0 > F2 ? F0 : F2) : (F1 > F2 ? F1 : F2);
-  static constexpr int NLDS = 5 * NPV + (9 * NG > 5 * NF ? 9 * NG : 5 * NF);
+  // primitives (5) and temperature per staged cell, then G or the faces
+  static constexpr int NLDS = 6 * NPV + (9 * NG > 5 * NF ? 9 * NG : 5 * NF);
   static constexpr int NPVS = NPV;   // primitive slots (stride of a variable)
 
   // slot of the cell (lx, ly, lz) whose coordinate along D lies in
@@ -669,10 +670,25 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
 // barrier every cell with 2 <= c <= n-3 along D takes -(F(c+1/2) -
 // F(c-1/2)) / vol.  Ends with a barrier (s_f is reused by the next axis).
 // ---------------------------------------------------------------------------
+// kxyz[D, :] of the left neighbour of cell c along D (the left cell of the
+// cell's own low face), loaded early so the face phase does not wait on it;
+// zero where that face does not exist.
+template <int D>
+__device__ __forceinline__ void load_kl_own(const BlockDesc& b, int64_t c, int gl_d, bool own, double kl[3]) {
+  const int64_t st = D == 0 ? (int64_t)b.nj * b.nk : (D == 1 ? (int64_t)b.nk : (int64_t)1);
+  const bool ok = own && gl_d >= 1;
+  const int64_t cl = ok ? c - st : c;   // always a valid address: a fixed number of loads
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double v = b.kxyz[(D * 3 + m) * b.N + cl];
+    kl[m] = ok ? v : 0.0;
+  }
+}
+
 template <int D, class T, int TI, int TJ, int TK, int NTH, int LIM>
 __device__ __forceinline__ void face_axis(const BlockDesc& b, int i0, int j0, int k0, const double* s_pv,
-                                          double* s_f, const double kk[9], bool own, int t, int x, int y, int z,
-                                          double rvol, double g, double inv[5]) {
+                                          double* s_f, const double kk[9], const double kl_own[3], bool own, int t,
+                                          int x, int y, int z, double rvol, double g, double inv[5]) {
   constexpr int NT = TI * TJ * TK;
   constexpr int Td = D == 0 ? TI : (D == 1 ? TJ : TK);
   constexpr int Pd = NT / Td;
@@ -696,11 +712,11 @@ __device__ __forceinline__ void face_axis(const BlockDesc& b, int i0, int j0, in
       for (int v = 0; v < 5; ++v) p4[s4][v] = s_pv[v * T::NPVS + sl];
     }
     const int64_t cr = (int64_t)ai * si + (int64_t)aj * sj + ak;
-    const int64_t cl = cr - st_d;
+    const int64_t cl = cr - st_d;    // a >= 1: in the block
     double kl[3], kr[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      kl[m] = b.kxyz[(D * 3 + m) * N + cl];
+      kl[m] = kr_own ? kl_own[m] : b.kxyz[(D * 3 + m) * N + cl];
       kr[m] = kr_own ? kk[D * 3 + m] : b.kxyz[(D * 3 + m) * N + cr];
     }
     double Fa[5];
@@ -729,14 +745,42 @@ __device__ __forceinline__ void face_axis(const BlockDesc& b, int i0, int j0, in
   __syncthreads();
 }
 
-template <int TI, int TJ, int TK, int LIM>
-__device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int j0, int k0,
-                                                 const double* __restrict__ qin, const double* q0, double* qout,
-                                                 const Gas& gas, double dt, int stage) {
+// Staging loads of one tile into registers: thread t's cells t, t + NT, ...
+// of the primitive slots.  Every thread issues the same number of loads
+// (indices past the last slot are clamped to it and the values unused), so
+// the compiler can wait for older loads without waiting for these.
+template <int TI, int TJ, int TK>
+__device__ __forceinline__ void fused_stage_loads(const BlockDesc& b, const double* __restrict__ qin, int i0, int j0,
+                                                  int k0, double qq[][5]) {
   using T = FusedGeom<TI, TJ, TK>;
+  constexpr int NIT = (T::NPV + T::NT - 1) / T::NT;
+  const int t = threadIdx.x;
+  const int64_t si = (int64_t)b.nj * b.nk, sj = b.nk;
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int h = t + it * T::NT;
+    const int3 lc = T::pv_cell(h < T::NPV ? h : T::NPV - 1);
+    const int64_t cc = (int64_t)wrap_to(i0 + lc.x, b.ni) * si + (int64_t)wrap_to(j0 + lc.y, b.nj) * sj +
+                       wrap_to(k0 + lc.z, b.nk);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * b.N + cc];
+  }
+}
+
+// One RK stage of one tile whose staging values are in qq (fused_stage_loads
+// of this tile).  Global loads are grouped by the phase that first needs
+// them: metrics of the G cells before the G phase, the faces' neighbour
+// metrics and volume before the faces, the state at the update.
+template <int TI, int TJ, int TK, int LIM>
+__device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0, int j0, int k0,
+                                                      const double* __restrict__ qin, const double* q0, double* qout,
+                                                      const Gas& gas, double dt, int stage, double qq[][5]) {
+  using T = FusedGeom<TI, TJ, TK>;
+  constexpr int NIT = (T::NPV + T::NT - 1) / T::NT;
   __shared__ double lds[T::NLDS];
   double* s_pv = lds;                    // [5][NPV]
-  double* s_g = lds + 5 * T::NPV;        // [9][NG]   (phase 2)
+  double* s_T = lds + 5 * T::NPV;        // [NPV]     temperature of the staged cells
+  double* s_g = lds + 6 * T::NPV;        // [9][NG]   (phase 2)
   double* s_f = s_g;                     // [5][NF]   (phase 3, aliases G)
   const int t = threadIdx.x;
   const int ni = b.ni, nj = b.nj, nk = b.nk;
@@ -749,62 +793,63 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
   const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
   const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
   const bool own = gi < ni && gj < nj && gk < nk;
-  const int64_t c = (int64_t)gi * si + (int64_t)gj * sj + gk;
   auto gidx_wrapped = [&](int lx, int ly, int lz) -> int64_t {
     return (int64_t)wrap_to(i0 + lx, ni) * si + (int64_t)wrap_to(j0 + ly, nj) * sj + wrap_to(k0 + lz, nk);
   };
+  // the tile cell of this thread (wrapped into the block when outside it)
+  const int64_t c = gidx_wrapped(x, y, z);
 
-  // ---- 1. primitives ------------------------------------------------------
-  for (int h = t; h < T::NPV; h += T::NT) {
-    const int3 lc = T::pv_cell(h);
-    const int lx = lc.x, ly = lc.y, lz = lc.z;
-    const int64_t cc = gidx_wrapped(lx, ly, lz);
-    double qq[5], pp[5];
+  // ---- 1. primitives from the staged registers --------------------------------
 #pragma unroll
-    for (int v = 0; v < 5; ++v) qq[v] = qin[v * N + cc];
-    cons_to_prim_fast(qq, g, pp);
+  for (int it = 0; it < NIT; ++it) {
+    const int h = t + it * T::NT;
+    if (h < T::NPV) {
+      double pp[5];
+      cons_to_prim_fast(qq[it], g, pp);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) s_pv[v * T::NPV + h] = pp[v];
+      for (int v = 0; v < 5; ++v) s_pv[v * T::NPV + h] = pp[v];
+      s_T[h] = temperature_fast(pp, gb.R);
+    }
+  }
+
+  // ---- metrics of the G cells --------------------------------------------------
+  double kk[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kk[m] = b.kxyz[m * N + c];
+  constexpr int NGIT = (T::NG + T::NT - 1) / T::NT;
+  double kh[NGIT > 1 ? NGIT - 1 : 1][9];
+#pragma unroll
+  for (int it = 1; it < NGIT; ++it) {
+    const int h = t + it * T::NT;
+    const int3 lc = T::g_cell(h < T::NG ? h : T::NG - 1);
+    const int64_t cc = gidx_wrapped(lc.x, lc.y, lc.z);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) kh[it - 1][m] = b.kxyz[m * N + cc];
   }
   __syncthreads();
 
   // ---- 2. viscous flux tensor and its divergence -----------------------------
-  double kk[9];
 #pragma unroll
-  for (int m = 0; m < 9; ++m) kk[m] = own ? b.kxyz[m * N + c] : 0.0;
-  for (int h = t; h < T::NG; h += T::NT) {
+  for (int it = 0; it < NGIT; ++it) {
+    const int h = t + it * T::NT;
+    if (h >= T::NG) break;
     const int3 lc = T::g_cell(h);
     const int lx = lc.x, ly = lc.y, lz = lc.z;
     double kc[9];
-    if (h < T::NT) {
 #pragma unroll
-      for (int m = 0; m < 9; ++m) kc[m] = kk[m];
-      if (!own) {
-        const int64_t cc = gidx_wrapped(lx, ly, lz);
-#pragma unroll
-        for (int m = 0; m < 9; ++m) kc[m] = b.kxyz[m * N + cc];
-      }
-    } else {
-      const int64_t cc = gidx_wrapped(lx, ly, lz);
-#pragma unroll
-      for (int m = 0; m < 9; ++m) kc[m] = b.kxyz[m * N + cc];
-    }
-    double pv[5], d[3][4];
+    for (int m = 0; m < 9; ++m) kc[m] = it == 0 ? kk[m] : kh[it > 0 ? it - 1 : 0][m];
+    double d[3][4];
     const int s0 = T::pv_slot(lx, ly, lz);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) pv[v] = s_pv[v * T::NPV + s0];
-    const double Tc = temperature_fast(pv, gb.R);
+    const double pv[5] = {0.0, s_pv[T::NPV + s0], s_pv[2 * T::NPV + s0], s_pv[3 * T::NPV + s0], 0.0};
+    const double Tc = s_T[s0];
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       const int sp = T::pv_slot(lx + (l == 0), ly + (l == 1), lz + (l == 2));
       const int sm = T::pv_slot(lx - (l == 0), ly - (l == 1), lz - (l == 2));
-      double pp[5], pm[5];
-#pragma unroll
-      for (int v = 0; v < 5; ++v) { pp[v] = s_pv[v * T::NPV + sp]; pm[v] = s_pv[v * T::NPV + sm]; }
-      d[l][0] = 0.5 * (pp[1] - pm[1]);
-      d[l][1] = 0.5 * (pp[2] - pm[2]);
-      d[l][2] = 0.5 * (pp[3] - pm[3]);
-      d[l][3] = 0.5 * (temperature_fast(pp, gb.R) - temperature_fast(pm, gb.R));
+      d[l][0] = 0.5 * (s_pv[T::NPV + sp] - s_pv[T::NPV + sm]);
+      d[l][1] = 0.5 * (s_pv[2 * T::NPV + sp] - s_pv[2 * T::NPV + sm]);
+      d[l][2] = 0.5 * (s_pv[3 * T::NPV + sp] - s_pv[3 * T::NPV + sm]);
+      d[l][3] = 0.5 * (s_T[sp] - s_T[sm]);
     }
     double g9[9];
     viscous_flux_fast(pv, Tc, d, kc, gb, g9);
@@ -833,11 +878,15 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
   __syncthreads();   // G space is reused by the face buffer
 
   // ---- 3. inviscid faces, axis by axis ---------------------------------------
+  double kl0[3], kl1[3], kl2[3];
+  load_kl_own<0>(b, c, gi, own, kl0);
+  load_kl_own<1>(b, c, gj, own, kl1);
+  load_kl_own<2>(b, c, gk, own, kl2);
   double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const double rvol = own ? rcp(b.vol[c] + 1e-300) : 0.0;
-  face_axis<0, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
-  face_axis<1, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
-  face_axis<2, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
+  face_axis<0, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, kl0, own, t, x, y, z, rvol, g, inv);
+  face_axis<1, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, kl1, own, t, x, y, z, rvol, g, inv);
+  face_axis<2, T, TI, TJ, TK, T::NT, LIM>(b, i0, j0, k0, s_pv, s_f, kk, kl2, own, t, x, y, z, rvol, g, inv);
   if (!own) return;
 
   // ---- 4. rhs and the RK3 combination ----------------------------------------
@@ -855,6 +904,16 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
   }
 #pragma unroll
   for (int v = 0; v < 5; ++v) qout[v * N + c] = rk_combine(stage, qq0[v], qs[v], dt, rhs[v]);
+}
+
+template <int TI, int TJ, int TK, int LIM>
+__device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int j0, int k0,
+                                                 const double* __restrict__ qin, const double* q0, double* qout,
+                                                 const Gas& gas, double dt, int stage) {
+  using T = FusedGeom<TI, TJ, TK>;
+  double qq[(T::NPV + T::NT - 1) / T::NT][5];
+  fused_stage_loads<TI, TJ, TK>(b, qin, i0, j0, k0, qq);
+  stage_tile_fused_regs<TI, TJ, TK, LIM>(b, i0, j0, k0, qin, q0, qout, gas, dt, stage, qq);
 }
 
 template <int TI, int TJ, int TK, int LIM>
@@ -963,18 +1022,32 @@ __device__ __forceinline__ void visc_tile(const BlockDesc& b, int i0, int j0, in
   const int64_t si = (int64_t)nj * nk, sj = nk;
   Gas gb = gas;
   gb.mu0 = b.mu0;
-  for (int h = t; h < T::NG; h += T::NT) {
-    const int3 lc = T::g_cell(h);
-    const int64_t cc =
-        (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
-    double qq[5], pp[5];
+  {
+    constexpr int NIT = (T::NG + T::NT - 1) / T::NT;
+    double qq[NIT][5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) qq[v] = qin[v * N + cc];
-    cons_to_prim_fast(qq, gas.gamma, pp);
-    s_w[h] = pp[1];
-    s_w[T::NG + h] = pp[2];
-    s_w[2 * T::NG + h] = pp[3];
-    s_w[3 * T::NG + h] = temperature_fast(pp, gb.R);
+    for (int it = 0; it < NIT; ++it) {
+      const int h = t + it * T::NT;
+      if (h < T::NG) {
+        const int3 lc = T::g_cell(h);
+        const int64_t cc =
+            (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * N + cc];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int h = t + it * T::NT;
+      if (h < T::NG) {
+        double pp[5];
+        cons_to_prim_fast(qq[it], gas.gamma, pp);
+        s_w[h] = pp[1];
+        s_w[T::NG + h] = pp[2];
+        s_w[2 * T::NG + h] = pp[3];
+        s_w[3 * T::NG + h] = temperature_fast(pp, gb.R);
+      }
+    }
   }
   __syncthreads();
   const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
@@ -1020,25 +1093,44 @@ __device__ __forceinline__ void stage_tile_split(const BlockDesc& b, int i0, int
   const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
   const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
   const bool own = t < T::NT && gi < ni && gj < nj && gk < nk;
-  const int64_t c = (int64_t)gi * si + (int64_t)gj * sj + gk;
+  // the thread's cell, wrapped into the block when outside it (a valid address)
+  const int64_t c = (int64_t)wrap_to(gi, ni) * si + (int64_t)wrap_to(gj, nj) * sj + wrap_to(gk, nk);
 
   // ---- 1. primitives of the tile and the two-cell cross halo ------------------
-  for (int h = t; h < T::NP; h += NTH) {
-    const int3 lc = T::p_cell(h);
-    const int64_t cc =
-        (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
-    double qq[5], pp[5];
+  {
+    constexpr int NIT = (T::NP + NTH - 1) / NTH;
+    double qq[NIT][5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) qq[v] = qin[v * N + cc];
-    cons_to_prim_fast(qq, g, pp);
+    for (int it = 0; it < NIT; ++it) {
+      const int h = t + it * NTH;
+      if (h < T::NP) {
+        const int3 lc = T::p_cell(h);
+        const int64_t cc =
+            (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) s_pv[v * T::NP + h] = pp[v];
+        for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * N + cc];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int h = t + it * NTH;
+      if (h < T::NP) {
+        double pp[5];
+        cons_to_prim_fast(qq[it], g, pp);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) s_pv[v * T::NP + h] = pp[v];
+      }
+    }
   }
 
   // ---- 2. divergence of the viscous tensor (G of the six wrapped neighbours) ---
   double kk[9];
 #pragma unroll
   for (int m = 0; m < 9; ++m) kk[m] = own ? b.kxyz[m * N + c] : 0.0;
+  double kl0[3], kl1[3], kl2[3];
+  load_kl_own<0>(b, c, gi, own, kl0);
+  load_kl_own<1>(b, c, gj, own, kl1);
+  load_kl_own<2>(b, c, gk, own, kl2);
   double vis[4] = {0.0, 0.0, 0.0, 0.0};
   if (own) {
     const int64_t cp[3] = {c + (int64_t)(wrap_p(gi, ni) - gi) * si, c + (int64_t)(wrap_p(gj, nj) - gj) * sj,
@@ -1065,9 +1157,9 @@ __device__ __forceinline__ void stage_tile_split(const BlockDesc& b, int i0, int
   // ---- 3. inviscid faces, axis by axis ---------------------------------------
   double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const double rvol = own ? rcp(b.vol[c] + 1e-300) : 0.0;
-  face_axis<0, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
-  face_axis<1, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
-  face_axis<2, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, own, t, x, y, z, rvol, g, inv);
+  face_axis<0, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, kl0, own, t, x, y, z, rvol, g, inv);
+  face_axis<1, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, kl1, own, t, x, y, z, rvol, g, inv);
+  face_axis<2, T, TI, TJ, TK, NTH, LIM>(b, i0, j0, k0, s_pv, s_f, kk, kl2, own, t, x, y, z, rvol, g, inv);
   if (!own) return;
 
   // ---- 4. rhs and the RK3 combination ----------------------------------------
@@ -1125,15 +1217,16 @@ __global__ void __launch_bounds__(NTH) k_stage_split_batched(const BlockDesc* __
   stage_tile_split<TI, TJ, TK, NTH, LIM>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, dt, stage);
 }
 
-// Stage-kernel variants the autotuner chooses from.  Fused kernel:
+// Stage-kernel variants the autotuner chooses from (ids 0-4 fused, 5-9
+// split pair).  Fused kernel:
 // (id, TI, TJ, TK), one thread per cell.  Split pair: (id, TI, TJ, TK,
 // threads of the stage kernel): with one thread per cell an axis's faces
 // take two passes, with a thread per face (NTH1) one.
 #define FAMRDP_TILE_VARIANTS(X) \
-  X(0, 4, 4, 16)                \
+  X(0, 8, 4, 8)                 \
   X(1, 4, 8, 16)                \
   X(2, 8, 4, 16)                \
-  X(3, 4, 4, 32)                \
+  X(3, 4, 8, 8)                 \
   X(4, 8, 8, 8)
 #define FAMRDP_SPLIT_VARIANTS(X)  \
   X(5, 4, 4, 16, 256)             \
@@ -1159,7 +1252,8 @@ inline void tile_dims(int v, int& ti, int& tj, int& tk) {
 // One stage of one block (plain launch) with variant v; the limiter is a
 // template parameter of the stage kernels (LIM_MINMOD / LIM_VANLEER).
 template <int LIM>
-inline void launch_stage_lim(int v, const BlockDesc& d, int stage, const Gas& gas, double dt, hipStream_t st) {
+inline void launch_stage_lim(int v, const BlockDesc& d, int stage, const Gas& gas, double dt, int ncu,
+                             hipStream_t st) {
   int TI, TJ, TK;
   tile_dims(v, TI, TJ, TK);
   const int nti = (d.ni + TI - 1) / TI, ntj = (d.nj + TJ - 1) / TJ, ntk = (d.nk + TK - 1) / TK;
@@ -1179,16 +1273,17 @@ inline void launch_stage_lim(int v, const BlockDesc& d, int stage, const Gas& ga
   }
 }
 
-inline void launch_fused(int v, const BlockDesc& d, int stage, const Gas& gas, int lim, double dt, hipStream_t st) {
-  if (lim == LIM_VANLEER) launch_stage_lim<LIM_VANLEER>(v, d, stage, gas, dt, st);
-  else launch_stage_lim<LIM_MINMOD>(v, d, stage, gas, dt, st);
+inline void launch_fused(int v, const BlockDesc& d, int stage, const Gas& gas, int lim, double dt, int ncu,
+                         hipStream_t st) {
+  if (lim == LIM_VANLEER) launch_stage_lim<LIM_VANLEER>(v, d, stage, gas, dt, ncu, st);
+  else launch_stage_lim<LIM_MINMOD>(v, d, stage, gas, dt, ncu, st);
 }
 
 // One stage of every block in one launch (two for the split pair) over a tile
 // table built for v.
 template <int LIM>
 inline void launch_stage_batched_lim(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
-                                     const Gas& gas, double dt, hipStream_t st) {
+                                     const Gas& gas, double dt, int ncu, hipStream_t st) {
   switch (v) {
 #define FAMRDP_TB(id, a, b, c) \
   case id: k_stage_fused_batched<a, b, c, LIM><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, dt); return;
@@ -1205,9 +1300,9 @@ inline void launch_stage_batched_lim(int v, const BlockDesc* blocks, const Tile3
 }
 
 inline void launch_fused_batched(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
-                                 const Gas& gas, int lim, double dt, hipStream_t st) {
-  if (lim == LIM_VANLEER) launch_stage_batched_lim<LIM_VANLEER>(v, blocks, tiles, ntiles, stage, gas, dt, st);
-  else launch_stage_batched_lim<LIM_MINMOD>(v, blocks, tiles, ntiles, stage, gas, dt, st);
+                                 const Gas& gas, int lim, double dt, int ncu, hipStream_t st) {
+  if (lim == LIM_VANLEER) launch_stage_batched_lim<LIM_VANLEER>(v, blocks, tiles, ntiles, stage, gas, dt, ncu, st);
+  else launch_stage_batched_lim<LIM_MINMOD>(v, blocks, tiles, ntiles, stage, gas, dt, ncu, st);
 }
 
 }  // namespace famrdp

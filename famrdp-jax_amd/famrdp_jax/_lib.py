@@ -87,6 +87,10 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"famrdp HIP library not built: {path} is missing (run __graft_entry__.build())")
+    # torch first: its bundled HIP runtime (soname libamdhip64.so.7) then also
+    # serves this library.  Loaded the other way round, the process would hold
+    # two HIP runtimes and torch's kernels would fail to launch.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(path)
     for name, (args, res) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -237,12 +237,14 @@ def main():
     from famrdp_jax import _lib
     _lib.load()
     c2 = bench_c2(args.n, args.steps, args.warmup, tile=args.tile)
+    # HBM bytes per stage launch of the chosen variant, measured with rocprofv3
+    # counters for this library version and size (profiles/stage_kernel_traffic.json)
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
             t = json.load(open(TRAFFIC_FILE))
             if t.get("version") == _lib.lib().famrdp_version().decode() and t.get("n") == args.n:
-                traffic = t.get("hbm_bytes_per_stage")
+                traffic = t.get("hbm_bytes_per_stage", {}).get(str(c2["variant"]))
         except Exception:
             traffic = None
     roofline = dict(bound="hbm", achieved=round(c2["achieved_gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",

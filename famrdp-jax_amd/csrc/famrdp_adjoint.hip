@@ -223,7 +223,7 @@ __device__ __forceinline__ double stage_cmu(int stage, double dt) {
 
 __global__ void __launch_bounds__(256) k_vjp_faces(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
                                                    int stage, double g, int lim, double dt) {
-  const Tile tl = tiles[blockIdx.x];
+  const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
   const BlockDesc& b = blocks[tl.block];
   const int64_t c = (int64_t)tl.start + threadIdx.x;
   if (c >= b.N) return;
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(256) k_vjp_faces(const BlockDesc* __restrict__
 
 __global__ void __launch_bounds__(256) k_vjp_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
                                                   int stage, Gas gas, double dt) {
-  const Tile tl = tiles[blockIdx.x];
+  const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
   const BlockDesc& b = blocks[tl.block];
   const int64_t c = (int64_t)tl.start + threadIdx.x;
   if (c >= b.N) return;
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) k_vjp_visc(const BlockDesc* __restrict__ 
 
 __global__ void __launch_bounds__(256) k_vjp_cell(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
                                                   int stage, Gas gas, int lim) {
-  const Tile tl = tiles[blockIdx.x];
+  const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
   const BlockDesc& b = blocks[tl.block];
   const int64_t c = (int64_t)tl.start + threadIdx.x;
   if (c >= b.N) return;

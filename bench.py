@@ -188,7 +188,7 @@ def bench_adjoint():
     from famrdp_jax.solver.stepper import state_to_device
     res = {}
     state, cfg, dt = setup_curvilinear(*cases.case_c4(96))
-    adj = Adjoint(state, cfg)
+    adj = Adjoint(state, cfg, autotune=True)
     res["c4"] = _time_adjoint(adj, dt, 50, OBJ_WALL_PRESSURE, 10)
     log(f"C4 adjoint: {res['c4']}")
     del adj, state
@@ -197,7 +197,7 @@ def bench_adjoint():
     state, cfg, mu0 = cases.ensemble(pairs)
     dev = state_to_device(state)
     dts = [cfl_dt(b.q, m, c)[0] for b, m, (_, c) in zip(dev.blocks, dev.metrics, pairs)]
-    adj = Adjoint(dev, cfg, batched=True)
+    adj = Adjoint(dev, cfg, batched=True, autotune=True)
     adj.set_block_params(mu0, dts)
     res["c5"] = _time_adjoint(adj, 1.0, 30, OBJ_KINETIC_ENERGY, 10)
     res["c5"]["cases"] = len(pairs)

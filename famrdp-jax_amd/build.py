@@ -5,7 +5,9 @@
 Compiles csrc/famrdp_capi.hip (which includes the kernels) into
 famrdp_jax/_lib/libfamrdp_hip.so with hipcc.  Packed FP32 instructions are
 disabled (the kernels are FP64; this keeps the compiler from forming them in
-any FP32 helper code).
+any FP32 helper code).  The machine scheduler favours instruction-level
+parallelism (max-ilp): the stage kernels are issue-bound FP64 code at two
+workgroups per CU, and this schedule measured 2 % faster on them.
 """
 from __future__ import annotations
 
@@ -18,6 +20,7 @@ OUT = os.path.join(HERE, "famrdp_jax", "_lib", "libfamrdp_hip.so")
 SRC = os.path.join(HERE, "csrc", "famrdp_capi.hip")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-sched-strategy=max-ilp",
          "-I" + os.path.join(HERE, "csrc")]
 
 

@@ -705,17 +705,24 @@ int famrdp_plan_set_block_params(famrdp_plan* p, const double* mu0, const double
   return FAMRDP_OK;
 }
 
-int famrdp_step_vjp(famrdp_plan* p, double* const* q_start, const double* const* kxyz, const double* const* vol,
-                    double* const* lam, double dt, double* dqinf, int32_t flags, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Reverse of one step; the stage-1/2 states are recomputed from q_start, or
+// copied from q1s/q2s when the caller kept them from the forward step.
+int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q1s, const double* const* q2s,
+                  const double* const* kxyz, const double* const* vol, double* const* lam, double dt, double* dqinf,
+                  int32_t flags, void* stream) {
   if (!p || !q_start || !kxyz || !vol || !lam) return fail(FAMRDP_E_INVALID, "step_vjp: bad arguments");
   if (!p->has_adjoint) return fail(FAMRDP_E_INVALID, "step_vjp: plan was created without FAMRDP_PLAN_ADJOINT");
   for (int b = 0; b < p->nblocks; ++b)
-    if (!kxyz[b] || !vol[b] || !lam[b] || !q_start[b])
+    if (!kxyz[b] || !vol[b] || !lam[b] || !q_start[b] || (q1s && (!q1s[b] || !q2s[b])))
       return fail(FAMRDP_E_INVALID, "step_vjp: NULL pointer for block " + std::to_string(b));
   hipStream_t st = S(stream);
   int rc = bind(p, q_start, kxyz, vol, st);
   if (rc) return rc;
-  // ---- recompute: q0' = BC(halo(q_start)) into qh, then stages 1 and 2
+  // ---- q0' = BC(halo(q_start)) into qh (BC snapshots kept), then stages 1-2
   for (int b = 0; b < p->nblocks; ++b) {
     k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].qh, q_start[b], 5 * p->N[b]);
     LAUNCH_CHECK();
@@ -724,8 +731,17 @@ int famrdp_step_vjp(famrdp_plan* p, double* const* q_start, const double* const*
   if (rc) return rc;
   rc = launch_bc(p, st, p->d_blocks_adj, true);
   if (rc) return rc;
-  rc = launch_stages(p, dt, flags, st, true, 2);
-  if (rc) return rc;
+  if (q1s) {
+    for (int b = 0; b < p->nblocks; ++b) {
+      k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].q1, q1s[b], 5 * p->N[b]);
+      LAUNCH_CHECK();
+      k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].q2, q2s[b], 5 * p->N[b]);
+      LAUNCH_CHECK();
+    }
+  } else {
+    rc = launch_stages(p, dt, flags, st, true, 2);
+    if (rc) return rc;
+  }
   // ---- reverse the three stages
   for (int b = 0; b < p->nblocks; ++b) {
     k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].la, lam[b], 5 * p->N[b]);
@@ -760,6 +776,36 @@ int famrdp_step_vjp(famrdp_plan* p, double* const* q_start, const double* const*
   }
   for (int b = 0; b < p->nblocks; ++b) {
     k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(lam[b], p->h_blocks_adj[b].l0, 5 * p->N[b]);
+    LAUNCH_CHECK();
+  }
+  return FAMRDP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int famrdp_step_vjp(famrdp_plan* p, double* const* q_start, const double* const* kxyz, const double* const* vol,
+                    double* const* lam, double dt, double* dqinf, int32_t flags, void* stream) {
+  return step_vjp_impl(p, q_start, nullptr, nullptr, kxyz, vol, lam, dt, dqinf, flags, stream);
+}
+
+int famrdp_step_vjp_stages(famrdp_plan* p, double* const* q_start, const double* const* q1, const double* const* q2,
+                           const double* const* kxyz, const double* const* vol, double* const* lam, double dt,
+                           double* dqinf, int32_t flags, void* stream) {
+  if (!q1 || !q2) return fail(FAMRDP_E_INVALID, "step_vjp_stages: stage-state arrays are NULL");
+  return step_vjp_impl(p, q_start, q1, q2, kxyz, vol, lam, dt, dqinf, flags, stream);
+}
+
+int famrdp_plan_stage_states(famrdp_plan* p, double* const* q1_out, double* const* q2_out, void* stream) {
+  if (!p || !q1_out || !q2_out) return fail(FAMRDP_E_INVALID, "stage_states: bad arguments");
+  if (!p->has_workspace) return fail(FAMRDP_E_INVALID, "stage_states: plan has no stage workspace");
+  hipStream_t st = S(stream);
+  for (int b = 0; b < p->nblocks; ++b) {
+    if (!q1_out[b] || !q2_out[b]) return fail(FAMRDP_E_INVALID, "stage_states: NULL pointer for block " + std::to_string(b));
+    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(q1_out[b], p->h_blocks[b].q1, 5 * p->N[b]);
+    LAUNCH_CHECK();
+    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(q2_out[b], p->h_blocks[b].q2, 5 * p->N[b]);
     LAUNCH_CHECK();
   }
   return FAMRDP_OK;

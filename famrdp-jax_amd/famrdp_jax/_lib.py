@@ -70,6 +70,9 @@ SIGNATURES = {
     "famrdp_plan_set_block_params": ([vp, ctypes.POINTER(f64), ctypes.POINTER(f64)], i32),
     "famrdp_step_vjp": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, vp,
                          i32, vp], i32),
+    "famrdp_step_vjp_stages": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                ctypes.POINTER(vp), ctypes.POINTER(vp), f64, vp, i32, vp], i32),
+    "famrdp_plan_stage_states": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp], i32),
     "famrdp_objective": ([i32, vp, vp, vp, i32, i32, i32, i32, f64, vp, vp, vp, vp], i32),
     "famrdp_halo_exchange": ([vp, ctypes.POINTER(vp), vp], i32),
     "famrdp_apply_bc": ([vp, ctypes.POINTER(vp), vp], i32),

@@ -148,11 +148,23 @@ class Plan:
         L.check(self._lib.famrdp_plan_timing(self._h, 0, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
-    def step_vjp(self, q_start, kxyzs, vols, lams, dt, dqinf=None, *, batched=False, direct=False):
-        """In place: lams (cotangent of the step's result) -> cotangent of q_start."""
+    def step_vjp(self, q_start, kxyzs, vols, lams, dt, dqinf=None, *, batched=False, direct=False, stages=None):
+        """In place: lams (cotangent of the step's result) -> cotangent of
+        q_start.  ``stages`` = (q1s, q2s) saved by stage_states after the
+        forward step from q_start; without it they are recomputed."""
         flags = (1 if batched else 0) | (2 if direct else 0)
-        L.check(self._lib.famrdp_step_vjp(self._h, self._ptrs(q_start), self._ptrs(kxyzs), self._ptrs(vols),
-                                          self._ptrs(lams), float(dt), L.ptr(dqinf), flags, L.stream()))
+        if stages is None:
+            L.check(self._lib.famrdp_step_vjp(self._h, self._ptrs(q_start), self._ptrs(kxyzs), self._ptrs(vols),
+                                              self._ptrs(lams), float(dt), L.ptr(dqinf), flags, L.stream()))
+        else:
+            q1s, q2s = stages
+            L.check(self._lib.famrdp_step_vjp_stages(self._h, self._ptrs(q_start), self._ptrs(q1s), self._ptrs(q2s),
+                                                     self._ptrs(kxyzs), self._ptrs(vols), self._ptrs(lams), float(dt),
+                                                     L.ptr(dqinf), flags, L.stream()))
+
+    def stage_states(self, q1s, q2s):
+        """Copy the stage-1/2 states of the last step into q1s, q2s."""
+        L.check(self._lib.famrdp_plan_stage_states(self._h, self._ptrs(q1s), self._ptrs(q2s), L.stream()))
 
     def set_block_params(self, mu0=None, dt_scale=None):
         arr = lambda x: None if x is None else (L.f64 * self.nblocks)(*[float(v) for v in x])

@@ -24,10 +24,14 @@ OBJ_KINETIC_ENERGY = 0
 OBJ_WALL_PRESSURE = 1
 
 
+def torch_empty_like(t):
+    return L._torch().empty_like(t)
+
+
 class Adjoint:
     """Forward and reverse sweeps for one configuration (all blocks resident)."""
 
-    def __init__(self, state: State, cfg, *, batched=False, direct=False):
+    def __init__(self, state: State, cfg, *, batched=False, direct=False, autotune=False):
         torch = L._torch()
         L.require_gpu()
         self.cfg = cfg
@@ -39,6 +43,8 @@ class Adjoint:
         self.plan = Plan(dims, cuts_from_topology(cfg.topology), bcs_from_config(cfg.topology, cfg.bc_params),
                          cfg.gas, cfg.scheme.limiter, cfg.ghost, workspace=True, adjoint=True)
         self.nb = len(self.q0)
+        if autotune:   # stage-kernel variant for these blocks (state untouched)
+            self.plan.autotune(self.q0, self.kxyz, self.vol, 0.0, batched=batched)
         self._work = torch.empty(1024, dtype=torch.float64, device="cuda")
         self._J = torch.empty(1, dtype=torch.float64, device="cuda")
 
@@ -80,19 +86,27 @@ class Adjoint:
             J += float(self._J.item())
         return J, grads
 
-    def reverse(self, lam, dt, nsteps, checkpoints, checkpoint_every, dqinf):
-        """lam (cotangent of the final state) -> cotangent of q0, in place."""
+    def reverse(self, lam, dt, nsteps, checkpoints, checkpoint_every, dqinf, *, keep_stages=True):
+        """lam (cotangent of the final state) -> cotangent of q0, in place.
+        Each segment is recomputed from its checkpoint keeping every step's
+        start state and (keep_stages) its stage-1/2 states, so the reverse of
+        a step does not recompute its stages."""
         k = checkpoint_every
         for seg_start in reversed(range(0, nsteps, k)):
             seg_end = min(seg_start + k, nsteps)
             cur = [c.clone() for c in checkpoints[seg_start]]
-            starts = []
+            starts, stages = [], []
             for _ in range(seg_start, seg_end):
                 starts.append([c.clone() for c in cur])
                 self._step(cur, dt)
+                if keep_stages:
+                    q1s = [torch_empty_like(c) for c in cur]
+                    q2s = [torch_empty_like(c) for c in cur]
+                    self.plan.stage_states(q1s, q2s)
+                    stages.append((q1s, q2s))
             for n in reversed(range(seg_end - seg_start)):
                 self.plan.step_vjp(starts[n], self.kxyz, self.vol, lam, dt, dqinf, batched=self.batched,
-                                   direct=self.direct)
+                                   direct=self.direct, stages=stages[n] if keep_stages else None)
         return lam
 
     def gradient(self, dt, nsteps, kind=OBJ_KINETIC_ENERGY, *, checkpoint_every=10, blocks=None, timing=None):

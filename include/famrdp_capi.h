@@ -167,6 +167,18 @@ int famrdp_plan_set_block_params(famrdp_plan* plan, const double* mu0, const dou
  * kernels).  Needs a plan created with FAMRDP_PLAN_ADJOINT. */
 int famrdp_step_vjp(famrdp_plan* plan, double* const* q_start, const double* const* kxyz, const double* const* vol,
                     double* const* lam, double dt, double* dqinf, int32_t flags, void* stream);
+/* famrdp_step_vjp with the step's stage-1 and stage-2 states supplied (per
+ * block, device 5*N doubles, as saved by famrdp_plan_stage_states right after
+ * the forward famrdp_step from q_start) instead of recomputed.  Same reverse
+ * sweep and results; no counterpart in the reference (checkpointed reverse
+ * passes that keep a segment's stage states). */
+int famrdp_step_vjp_stages(famrdp_plan* plan, double* const* q_start, const double* const* q1, const double* const* q2,
+                           const double* const* kxyz, const double* const* vol, double* const* lam, double dt,
+                           double* dqinf, int32_t flags, void* stream);
+/* Copies the stage-1 and stage-2 states of the plan's last famrdp_step (its
+ * workspace) into q1_out / q2_out (per block, device 5*N doubles).  No
+ * counterpart in the reference. */
+int famrdp_plan_stage_states(famrdp_plan* plan, double* const* q1_out, double* const* q2_out, void* stream);
 
 /* Build-defined objectives for the differentiation workloads (the reference
  * defines none): kind 0 = interior kinetic energy sum 1/2 rho|u|^2 vol,

@@ -165,6 +165,26 @@ def test_central_finite_differences_field_and_mach():
     assert abs(fdM - adM) <= 1e-6 * abs(adM)
 
 
+def test_kept_stage_states_equal_recomputed():
+    """The reverse pass that keeps each step's stage-1/2 states
+    (famrdp_plan_stage_states + famrdp_step_vjp_stages) gives the same bits as
+    the one that recomputes them (famrdp_step_vjp)."""
+    print("case: C4-like 12^3, 3 steps, kept vs recomputed stage states", flush=True)
+    state, cfg, dt = _c4_small(12)
+    adj = Adjoint(to_dev(state), cfg)
+    final, ckpts = adj.forward_checkpointed(adj.q0, dt, 3, 2)
+    J, lam0 = adj.objective(final, OBJ_WALL_PRESSURE)
+    outs = []
+    for keep in (True, False):
+        lam = [l.clone() for l in lam0]
+        dq = torch.zeros(6 * 5, dtype=torch.float64, device="cuda")
+        adj.reverse(lam, dt, 3, ckpts, 2, dq, keep_stages=keep)
+        torch.cuda.synchronize()
+        outs.append((lam[0].cpu().numpy(), dq.cpu().numpy()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
 def test_torch_autograd_function():
     print("case: torch.autograd.Function wrapper", flush=True)
     st, cfg = cases.case_c1(n=8, seed=9, p_noise=1e-3)

@@ -107,7 +107,7 @@ def bench_c2(n, steps, warmup, timing_steps=3, tile=-1):
                 achieved_gbs=achieved, finite=finite)
 
 
-def cpu_baseline(n=48, steps=6):
+def cpu_baseline(n=80, steps=10):
     """Oracle step (numpy, one core) on the same curvilinear TGV workload at
     n^3 interior: 1 warm-up step, then ``steps`` timed steps."""
     import numpy as np

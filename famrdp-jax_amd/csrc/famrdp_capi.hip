@@ -324,10 +324,11 @@ int famrdp_limiter(const double* a, const double* b, double* out, int64_t n, int
   return FAMRDP_OK;
 }
 
-int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, int64_t n, void* stream) {
+int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, double* out_rsqrt, int64_t n,
+                          void* stream) {
   if (n < 0 || (n > 0 && (!x || !out_rcp || !out_sqrt))) return fail(FAMRDP_E_INVALID, "fastmath_probe: bad arguments");
   if (n == 0) return FAMRDP_OK;
-  k_fastmath_probe<<<nblk(n), 256, 0, S(stream)>>>(x, out_rcp, out_sqrt, n);
+  k_fastmath_probe<<<nblk(n), 256, 0, S(stream)>>>(x, out_rcp, out_sqrt, out_rsqrt, n);
   LAUNCH_CHECK();
   return FAMRDP_OK;
 }

@@ -256,9 +256,13 @@ __global__ void __launch_bounds__(256) k_viscous_flux_fast(const double* __restr
 }
 
 __global__ void k_fastmath_probe(const double* __restrict__ x, double* __restrict__ r, double* __restrict__ s,
-                                 int64_t n) {
+                                 double* __restrict__ q, int64_t n) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t < n) { r[t] = rcp(x[t]); s[t] = fsqrt(x[t]); }
+  if (t < n) {
+    r[t] = rcp(x[t]);
+    s[t] = fsqrt(x[t]);
+    if (q) q[t] = rsq_acc(x[t]);
+  }
 }
 
 __global__ void __launch_bounds__(256) k_stage(const double* __restrict__ qin, const double* q0,
@@ -584,6 +588,10 @@ struct FusedGeom {
   static constexpr int NF = F0 > F1 ? (F0 > F2 ? F0 : F2) : (F1 > F2 ? F1 : F2);
   // primitives (5) and temperature per staged cell, then G or the faces
   static constexpr int NLDS = 6 * NPV + (9 * NG > 5 * NF ? 9 * NG : 5 * NF);
+  // re-staging form: G overwrites the primitives (which are staged again for
+  // the faces), so LDS holds the larger of the three phases only
+  static constexpr int NLDS_RS_A = 6 * NPV > 9 * NG ? 6 * NPV : 9 * NG;
+  static constexpr int NLDS_RS = NLDS_RS_A > 5 * NPV + 5 * NF ? NLDS_RS_A : 5 * NPV + 5 * NF;
   static constexpr int NPVS = NPV;   // primitive slots (stride of a variable)
 
   // slot of the cell (lx, ly, lz) whose coordinate along D lies in
@@ -771,17 +779,21 @@ __device__ __forceinline__ void fused_stage_loads(const BlockDesc& b, const doub
 // of this tile).  Global loads are grouped by the phase that first needs
 // them: metrics of the G cells before the G phase, the faces' neighbour
 // metrics and volume before the faces, the state at the update.
-template <int TI, int TJ, int TK, int LIM>
+// RS (re-staging): the viscous tensors are kept in registers until every
+// thread is done reading the primitives, then written over them; after the
+// divergence the primitives are staged again (their loads issued before the
+// divergence) for the faces.  Less LDS per workgroup, so more share a CU.
+template <int TI, int TJ, int TK, int LIM, bool RS>
 __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0, int j0, int k0,
                                                       const double* __restrict__ qin, const double* q0, double* qout,
                                                       const Gas& gas, double dt, int stage, double qq[][5]) {
   using T = FusedGeom<TI, TJ, TK>;
   constexpr int NIT = (T::NPV + T::NT - 1) / T::NT;
-  __shared__ double lds[T::NLDS];
-  double* s_pv = lds;                    // [5][NPV]
-  double* s_T = lds + 5 * T::NPV;        // [NPV]     temperature of the staged cells
-  double* s_g = lds + 6 * T::NPV;        // [9][NG]   (phase 2)
-  double* s_f = s_g;                     // [5][NF]   (phase 3, aliases G)
+  __shared__ double lds[RS ? T::NLDS_RS : T::NLDS];
+  double* s_pv = lds;                                  // [5][NPV]
+  double* s_T = lds + 5 * T::NPV;                      // [NPV]   temperature of the staged cells
+  double* s_g = RS ? lds : lds + 6 * T::NPV;           // [9][NG] (phase 2)
+  double* s_f = RS ? lds + 5 * T::NPV : s_g;           // [5][NF] (phase 3)
   const int t = threadIdx.x;
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
@@ -829,6 +841,7 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
   __syncthreads();
 
   // ---- 2. viscous flux tensor and its divergence -----------------------------
+  double gsave[RS ? NGIT : 1][9];
 #pragma unroll
   for (int it = 0; it < NGIT; ++it) {
     const int h = t + it * T::NT;
@@ -854,9 +867,24 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
     double g9[9];
     viscous_flux_fast(pv, Tc, d, kc, gb, g9);
 #pragma unroll
-    for (int m = 0; m < 9; ++m) s_g[m * T::NG + h] = g9[m];
+    for (int m = 0; m < 9; ++m) {
+      if constexpr (RS) gsave[it][m] = g9[m];
+      else s_g[m * T::NG + h] = g9[m];
+    }
   }
   __syncthreads();
+  if constexpr (RS) {   // the primitives are no longer read: G goes over them
+#pragma unroll
+    for (int it = 0; it < NGIT; ++it) {
+      const int h = t + it * T::NT;
+      if (h < T::NG) {
+#pragma unroll
+        for (int m = 0; m < 9; ++m) s_g[m * T::NG + h] = gsave[it][m];
+      }
+    }
+    fused_stage_loads<TI, TJ, TK>(b, qin, i0, j0, k0, qq);   // for the second staging
+    __syncthreads();
+  }
   double vis[4] = {0.0, 0.0, 0.0, 0.0};
   if (own) {
     const int sp[3] = {T::g_slot(x + 1, y, z), T::g_slot(x, y + 1, z), T::g_slot(x, y, z + 1)};
@@ -875,7 +903,20 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
       vis[r] = res;
     }
   }
-  __syncthreads();   // G space is reused by the face buffer
+  __syncthreads();   // G space is reused by the face buffer (and the primitives)
+  if constexpr (RS) {
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int h = t + it * T::NT;
+      if (h < T::NPV) {
+        double pp[5];
+        cons_to_prim_fast(qq[it], g, pp);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) s_pv[v * T::NPV + h] = pp[v];
+      }
+    }
+    __syncthreads();
+  }
 
   // ---- 3. inviscid faces, axis by axis ---------------------------------------
   double kl0[3], kl1[3], kl2[3];
@@ -906,35 +947,36 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
   for (int v = 0; v < 5; ++v) qout[v * N + c] = rk_combine(stage, qq0[v], qs[v], dt, rhs[v]);
 }
 
-template <int TI, int TJ, int TK, int LIM>
+template <int TI, int TJ, int TK, int LIM, bool RS>
 __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int j0, int k0,
                                                  const double* __restrict__ qin, const double* q0, double* qout,
                                                  const Gas& gas, double dt, int stage) {
   using T = FusedGeom<TI, TJ, TK>;
   double qq[(T::NPV + T::NT - 1) / T::NT][5];
   fused_stage_loads<TI, TJ, TK>(b, qin, i0, j0, k0, qq);
-  stage_tile_fused_regs<TI, TJ, TK, LIM>(b, i0, j0, k0, qin, q0, qout, gas, dt, stage, qq);
+  stage_tile_fused_regs<TI, TJ, TK, LIM, RS>(b, i0, j0, k0, qin, q0, qout, gas, dt, stage, qq);
 }
 
-template <int TI, int TJ, int TK, int LIM>
-__global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused(BlockDesc b, int stage, Gas gas, double dt, int ntj,
-                                                              int ntk) {
+// The re-staging form targets three workgroups per CU: at most 168 VGPRs.
+template <int TI, int TJ, int TK, int LIM, bool RS>
+__global__ void __launch_bounds__(TI * TJ * TK) __attribute__((amdgpu_waves_per_eu(RS ? 3 : 1)))
+k_stage_fused(BlockDesc b, int stage, Gas gas, double dt, int ntj, int ntk) {
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-  stage_tile_fused<TI, TJ, TK, LIM>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, dt, stage);
+  stage_tile_fused<TI, TJ, TK, LIM, RS>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, dt, stage);
 }
 
-template <int TI, int TJ, int TK, int LIM>
-__global__ void __launch_bounds__(TI * TJ * TK) k_stage_fused_batched(const BlockDesc* __restrict__ blocks,
-                                                                      const Tile3* __restrict__ tiles, int stage,
-                                                                      Gas gas, double dt) {
+template <int TI, int TJ, int TK, int LIM, bool RS>
+__global__ void __launch_bounds__(TI * TJ * TK) __attribute__((amdgpu_waves_per_eu(RS ? 3 : 1)))
+k_stage_fused_batched(const BlockDesc* __restrict__ blocks, const Tile3* __restrict__ tiles, int stage, Gas gas,
+                      double dt) {
   const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
   const BlockDesc& b = blocks[tl.block];
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-  stage_tile_fused<TI, TJ, TK, LIM>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, dt, stage);
+  stage_tile_fused<TI, TJ, TK, LIM, RS>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, dt, stage);
 }
 
 // ===========================================================================
@@ -1217,34 +1259,35 @@ __global__ void __launch_bounds__(NTH) k_stage_split_batched(const BlockDesc* __
   stage_tile_split<TI, TJ, TK, NTH, LIM>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, dt, stage);
 }
 
-// Stage-kernel variants the autotuner chooses from (ids 0-4 fused, 5-9
-// split pair).  Fused kernel:
-// (id, TI, TJ, TK), one thread per cell.  Split pair: (id, TI, TJ, TK,
+// Stage-kernel variants the autotuner chooses from (ids 0-4 and 10-12
+// fused, 5-9 split pair).  Fused kernel: (id, TI, TJ, TK, re-staging), one
+// thread per cell.  Split pair: (id, TI, TJ, TK,
 // threads of the stage kernel): with one thread per cell an axis's faces
 // take two passes, with a thread per face (NTH1) one.
 #define FAMRDP_TILE_VARIANTS(X) \
-  X(0, 8, 4, 8)                 \
-  X(1, 4, 8, 16)                \
-  X(2, 8, 4, 16)                \
-  X(3, 4, 8, 8)                 \
-  X(4, 8, 8, 8)
+  X(0, 8, 4, 8, false)          \
+  X(1, 4, 8, 16, false)         \
+  X(2, 8, 4, 16, false)         \
+  X(3, 4, 8, 8, false)          \
+  X(4, 4, 8, 8, true)           \
+  X(10, 8, 4, 8, true)          \
+  X(11, 8, 8, 4, true)          \
+  X(12, 4, 4, 16, true)
 #define FAMRDP_SPLIT_VARIANTS(X)  \
   X(5, 4, 4, 16, 256)             \
   X(6, 4, 8, 8, 256)              \
   X(7, 8, 8, 8, 512)              \
   X(8, 2, 8, 16, 256)             \
   X(9, 4, 8, 8, 320)
-constexpr int NUM_FUSED_VARIANTS = 5;
-constexpr int NUM_TILE_VARIANTS = 10;
+constexpr int NUM_FUSED_VARIANTS = 5;   // ids 0-4 and 10-12 are the fused kernel
+constexpr int NUM_TILE_VARIANTS = 13;
 
 inline void tile_dims(int v, int& ti, int& tj, int& tk) {
   switch (v) {
-#define FAMRDP_TD(id, a, b, c) case id: ti = a; tj = b; tk = c; return;
 #define FAMRDP_TD5(id, a, b, c, n) case id: ti = a; tj = b; tk = c; return;
-    FAMRDP_TILE_VARIANTS(FAMRDP_TD)
+    FAMRDP_TILE_VARIANTS(FAMRDP_TD5)
     FAMRDP_SPLIT_VARIANTS(FAMRDP_TD5)
 #undef FAMRDP_TD5
-#undef FAMRDP_TD
     default: ti = 4; tj = 4; tk = 16;
   }
 }
@@ -1259,8 +1302,8 @@ inline void launch_stage_lim(int v, const BlockDesc& d, int stage, const Gas& ga
   const int nti = (d.ni + TI - 1) / TI, ntj = (d.nj + TJ - 1) / TJ, ntk = (d.nk + TK - 1) / TK;
   const int nt = nti * ntj * ntk;
   switch (v) {
-#define FAMRDP_TL(id, a, b, c) \
-  case id: k_stage_fused<a, b, c, LIM><<<nt, a * b * c, 0, st>>>(d, stage, gas, dt, ntj, ntk); return;
+#define FAMRDP_TL(id, a, b, c, rs) \
+  case id: k_stage_fused<a, b, c, LIM, rs><<<nt, a * b * c, 0, st>>>(d, stage, gas, dt, ntj, ntk); return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TL)
 #undef FAMRDP_TL
 #define FAMRDP_SL(id, a, b, c, n)                                                       \
@@ -1285,8 +1328,10 @@ template <int LIM>
 inline void launch_stage_batched_lim(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
                                      const Gas& gas, double dt, int ncu, hipStream_t st) {
   switch (v) {
-#define FAMRDP_TB(id, a, b, c) \
-  case id: k_stage_fused_batched<a, b, c, LIM><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, dt); return;
+#define FAMRDP_TB(id, a, b, c, rs)                                                                       \
+  case id:                                                                                               \
+    k_stage_fused_batched<a, b, c, LIM, rs><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, dt); \
+    return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TB)
 #undef FAMRDP_TB
 #define FAMRDP_SB(id, a, b, c, n)                                                                  \

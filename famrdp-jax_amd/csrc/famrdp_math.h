@@ -198,6 +198,19 @@ DF double fsqrt(double x) {
   return x > 0.0 ? g : x;
 }
 
+// 1/sqrt(x) for x > 0 from the hardware seed: two coupled Newton steps on
+// (sqrt x, 1/(2 sqrt x)); within 2 ulp of 1/sqrt(x) (checked on the GPU).
+DF double rsq_acc(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  h = fma(h, r, h);
+  return 2.0 * h;
+}
+
 // van Leer with the division as a reciprocal (stage-kernel form)
 DF double vanleer_fast(double a, double b) {
   const double ab = a + b;
@@ -230,9 +243,11 @@ DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3],
   const double uR = pr[1], vR = pr[2], wR = pr[3], pR = pr[4];
   const double EL = pL * rgm1 + rhoL * (0.5 * (uL * uL + vL * vL + wL * wL));
   const double ER = pR * rgm1 + rhoR * (0.5 * (uR * uR + vR * vR + wR * wR));
-  const double HL = (EL + pL) * rcp(rhoL), HR = (ER + pR) * rcp(rhoR);
+  // 1/sqrt(rho) gives sqrt(rho), 1/rho and, below, 1/sqrt(rhoL rhoR)
+  const double isL = rsq_acc(rhoL), isR = rsq_acc(rhoR);
+  const double HL = (EL + pL) * (isL * isL), HR = (ER + pR) * (isR * isR);
 
-  const double sqL = fsqrt(rhoL), sqR = fsqrt(rhoR);
+  const double sqL = rhoL * isL, sqR = rhoR * isR;
   const double rs = rcp(fmax(sqL + sqR, 1e-150));
   const double uh = (sqL * uL + sqR * uR) * rs;
   const double vh = (sqL * vL + sqR * vR) * rs;
@@ -240,8 +255,9 @@ DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3],
   const double Hh = (sqL * HL + sqR * HR) * rs;
   const double q2 = uh * uh + vh * vh + wh * wh;
   const double c2 = fmax(gm1 * (Hh - 0.5 * q2), 1e-30);
-  const double rc2 = rcp(c2);
-  const double ch = fsqrt(c2);
+  const double isc = rsq_acc(c2);
+  const double rc2 = isc * isc;
+  const double ch = c2 * isc;
   const double unh = uh * n[0] + vh * n[1] + wh * n[2];
   const double rhoh = sqL * sqR;
 
@@ -262,7 +278,7 @@ DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3],
   const double a5 = (dp + rcd) * (0.5 * rc2);
   const double a23 = d0 - dp * rc2;
 
-  const double rrh = rcp(rhoh);
+  const double rrh = isL * isR;
   const double dut = (d1 - uh * d0) * rrh, dvt_ = (d2 - vh * d0) * rrh, dwt = (d3 - wh * d0) * rrh;
   const double dunf = dut * n[0] + dvt_ * n[1] + dwt * n[2];
   const double tx = dut - dunf * n[0], ty = dvt_ - dunf * n[1], tz = dwt - dunf * n[2];
@@ -297,8 +313,12 @@ DF void face_flux_prim(const double pv0[5], const double pv1[5], const double pv
     pR[v] = pv2[v] - 0.5 * sR;
   }
   const double nf0 = 0.5 * (k0[0] + k1[0]), nf1 = 0.5 * (k0[1] + k1[1]), nf2 = 0.5 * (k0[2] + k1[2]);
-  const double nm = fsqrt(nf0 * nf0 + nf1 * nf1 + nf2 * nf2) + 1e-300;
-  const double rn = rcp(nm);
+  // |n| + 1e-300 and its reciprocal from one 1/sqrt (|n| = 0: 1e-300, 1e300)
+  const double nn = nf0 * nf0 + nf1 * nf1 + nf2 * nf2;
+  const double isn = rsq_acc(nn);
+  const bool pos = nn > 0.0;
+  const double nm = (pos ? nn * isn : 0.0) + 1e-300;
+  const double rn = pos ? isn : 1e300;
   const double nu[3] = {nf0 * rn, nf1 * rn, nf2 * rn};
   double F[5];
   roe_flux_prim(pL, pR, nu, g, F);

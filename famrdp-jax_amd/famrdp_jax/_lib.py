@@ -38,9 +38,10 @@ class Bc(ctypes.Structure):
 
 # (name, argtypes) of every symbol in include/famrdp_capi.h
 # stage-kernel variants of famrdp_plan_autotune / famrdp_plan_set_tile:
-# 0-4 tile shapes of the fused kernel, 5-9 of the split pair
-NUM_TILE_VARIANTS = 10
-NUM_FUSED_VARIANTS = 5
+# 0-4 and 10-12 tile shapes of the fused kernel (10-12 and 4 re-stage the
+# primitives), 5-9 of the split pair
+NUM_TILE_VARIANTS = 13
+FUSED_VARIANTS = (0, 1, 2, 3, 4, 10, 11, 12)
 
 SIGNATURES = {
     "famrdp_last_error": ([], ctypes.c_char_p),
@@ -50,7 +51,7 @@ SIGNATURES = {
     "famrdp_sutherland": ([vp, vp, i64, f64, f64, f64, vp], i32),
     "famrdp_speed_of_sound": ([vp, vp, vp, i64, f64, vp], i32),
     "famrdp_limiter": ([vp, vp, vp, i64, i32, vp], i32),
-    "famrdp_fastmath_probe": ([vp, vp, vp, i64, vp], i32),
+    "famrdp_fastmath_probe": ([vp, vp, vp, vp, i64, vp], i32),
     "famrdp_muscl2": ([vp, vp, vp, i32, i32, i32, i32, i32, i32, vp], i32),
     "famrdp_roe_flux": ([vp, vp, vp, vp, i64, f64, f64, vp], i32),
     "famrdp_invscd_rhs": ([vp, vp, vp, vp, i32, i32, i32, f64, i32, vp], i32),

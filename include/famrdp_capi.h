@@ -100,10 +100,12 @@ int famrdp_axpby(double* out, double a, const double* x, double b, const double*
 int famrdp_compute_metrics(const double* xyz, double* jac, double* kxyz, double* vol, int32_t ni, int32_t nj,
                            int32_t nk, void* stream);
 
-/* The stage kernels' reciprocal and square root (hardware seed + Newton
- * refinement) on n values, for verifying their accuracy against IEEE 1/x and
- * sqrt(x).  No counterpart in the reference. */
-int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, int64_t n, void* stream);
+/* The stage kernels' reciprocal, square root and reciprocal square root
+ * (hardware seed + Newton refinement) on n values, for verifying their
+ * accuracy against IEEE 1/x, sqrt(x) and 1/sqrt(x) (out_rsqrt may be NULL).
+ * No counterpart in the reference. */
+int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, double* out_rsqrt, int64_t n,
+                          void* stream);
 
 /* ---- plan: one multi-block configuration (Config.topology + gas + scheme) - */
 
@@ -131,14 +133,15 @@ int famrdp_step(famrdp_plan* plan, double* const* q, const double* const* kxyz, 
  * flags (captured on first use, re-captured when any of them changes). */
 int famrdp_step_graph(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                       double dt, int32_t flags, int32_t nsteps, void* stream);
-/* Stage-kernel selection (no counterpart in the reference).  Variants 0-4 are
- * tile shapes of the fused stage kernel, 5-9 tile shapes of the split pair
+/* Stage-kernel selection (no counterpart in the reference).  Variants 0-4
+ * and 10-12 are tile shapes of the fused stage kernel (4 and 10-12 stage the
+ * primitives twice to need less LDS), 5-9 tile shapes of the split pair
  * (viscous tensor kernel + stage kernel).  famrdp_plan_autotune times `reps`
  * stage-1 launches of every variant for these arrays (stage 1 writes only the
  * plan's workspace, so the state is not modified), writes the per-variant ms
  * to ms_out (may be NULL; FAMRDP_NUM_TILE_VARIANTS entries) and keeps the
  * fastest. */
-#define FAMRDP_NUM_TILE_VARIANTS 10
+#define FAMRDP_NUM_TILE_VARIANTS 13
 int famrdp_plan_autotune(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                          double dt, int32_t flags, int32_t reps, double* ms_out, void* stream);
 int famrdp_plan_set_tile(famrdp_plan* plan, int32_t variant);

@@ -222,8 +222,9 @@ def test_cfl_dt_and_monitor():
 
 
 def test_fastmath_rcp_sqrt_accuracy():
-    """The stage kernels' reciprocal and square root stay within 2 ulp of the
-    IEEE results over the magnitudes the solver meets (and the 1e-300 floors)."""
+    """The stage kernels' reciprocal, square root and reciprocal square root
+    stay within 2 ulp of the IEEE results (1/x, sqrt(x), 1/sqrt(x)) over the
+    magnitudes the solver meets (and the 1e-300 floors)."""
     print("case: fastmath probe", flush=True)
     import ctypes
     import torch
@@ -232,14 +233,19 @@ def test_fastmath_rcp_sqrt_accuracy():
     x = np.concatenate([10.0 ** rng.uniform(-300, 300, 200000), rng.uniform(0.5, 2.0, 200000),
                         np.array([1e-300, 1e-150, 1e-30, 1.0, 2.0, 3.0, 0.1, 1.4 - 1.0])])
     xd = torch.from_numpy(x).cuda()
-    r = torch.empty_like(xd); s = torch.empty_like(xd)
-    L.check(L.lib().famrdp_fastmath_probe(xd.data_ptr(), r.data_ptr(), s.data_ptr(), x.size, L.stream()))
-    r = r.cpu().numpy(); s = s.cpu().numpy()
+    r = torch.empty_like(xd); s = torch.empty_like(xd); q = torch.empty_like(xd)
+    L.check(L.lib().famrdp_fastmath_probe(xd.data_ptr(), r.data_ptr(), s.data_ptr(), q.data_ptr(), x.size,
+                                          L.stream()))
+    r = r.cpu().numpy(); s = s.cpu().numpy(); q = q.cpu().numpy()
     ulp = np.spacing(1.0 / x)
     assert np.abs(r - 1.0 / x).max() <= 0 or (np.abs(r - 1.0 / x) / ulp).max() <= 2.0
     ulp_s = np.spacing(np.sqrt(x))
     assert (np.abs(s - np.sqrt(x)) / ulp_s).max() <= 2.0
+    rq = 1.0 / np.sqrt(x)
+    eq = (np.abs(q - rq) / np.spacing(rq)).max()
+    print(f"  rsqrt max error {eq:.2f} ulp", flush=True)
+    assert eq <= 2.0
     z = torch.zeros(4, dtype=torch.float64, device="cuda")
     r0 = torch.empty_like(z); s0 = torch.empty_like(z)
-    L.check(L.lib().famrdp_fastmath_probe(z.data_ptr(), r0.data_ptr(), s0.data_ptr(), 4, L.stream()))
+    L.check(L.lib().famrdp_fastmath_probe(z.data_ptr(), r0.data_ptr(), s0.data_ptr(), None, 4, L.stream()))
     assert (s0.cpu().numpy() == 0.0).all()

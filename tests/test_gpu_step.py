@@ -161,7 +161,7 @@ def test_tile_variants_bitwise_identical_and_autotune():
             st.advance(dt, 2, batched=(v % 2 == 1))
             torch.cuda.synchronize()
             out = [q.cpu().numpy() for q in st.qs]
-            form = "fused" if v < _lib.NUM_FUSED_VARIANTS else "split"
+            form = "fused" if v in _lib.FUSED_VARIANTS else "split"
             if form not in ref:
                 ref[form] = out
             else:

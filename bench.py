@@ -164,16 +164,20 @@ def bench_c3(steps=10, warmup=2):
 
 
 def _time_adjoint(adj, dt, nsteps, kind, k):
+    """forward alone vs forward + adjoint, checkpointing every k steps and with
+    every step's states kept in memory (k = 0)."""
     adj.forward(dt, 2)
     _sync()
     t0 = time.perf_counter()
     adj.forward(dt, nsteps)
     _sync()
     fwd = time.perf_counter() - t0
-    tm = {}
+    tm, tm0 = {}, {}
     adj.gradient(dt, nsteps, kind, checkpoint_every=k, timing=tm)
+    adj.gradient(dt, nsteps, kind, checkpoint_every=0, timing=tm0)
     return dict(steps=nsteps, checkpoint_every=k, forward_ms=fwd * 1e3, gradient_ms=tm["total_s"] * 1e3,
-                ratio=tm["total_s"] / fwd)
+                ratio=tm["total_s"] / fwd, gradient_all_states_in_memory_ms=tm0["total_s"] * 1e3,
+                ratio_all_states_in_memory=tm0["total_s"] / fwd)
 
 
 def bench_adjoint():

@@ -137,7 +137,7 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
   const int k = (int)(y % nk), j = (int)((y / nk) % nj), i = (int)(y / ((int64_t)nk * nj));
   double lpv[5] = {0, 0, 0, 0, 0};
   // ---- inviscid: face states and MUSCL slopes along each axis
-#pragma unroll 1
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double* fsd = b.fs + (int64_t)(d * 10) * N;
     const int yd = d == 0 ? i : (d == 1 ? j : k);
@@ -233,7 +233,8 @@ __global__ void __launch_bounds__(256) k_vjp_faces(const BlockDesc* __restrict__
   cell_vjp_faces(b, qs, lin, stage_cmu(stage, dt * b.dt_scale), c, g, lim);
 }
 
-__global__ void __launch_bounds__(256) k_vjp_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
+// at most 128 VGPRs: these memory-bound gathers want four waves per SIMD
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vjp_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
                                                   int stage, Gas gas, double dt) {
   const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
   const BlockDesc& b = blocks[tl.block];
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(256) k_vjp_visc(const BlockDesc* __restrict__ 
   cell_vjp_visc(b, qs, lin, stage_cmu(stage, dt * b.dt_scale), c, gas);
 }
 
-__global__ void __launch_bounds__(256) k_vjp_cell(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vjp_cell(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
                                                   int stage, Gas gas, int lim) {
   const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
   const BlockDesc& b = blocks[tl.block];

@@ -110,14 +110,35 @@ class Adjoint:
         return lam
 
     def gradient(self, dt, nsteps, kind=OBJ_KINETIC_ENERGY, *, checkpoint_every=10, blocks=None, timing=None):
-        """Returns (J, dJ/dq0 per block, dJ/dq_inf as (nblocks, 6, 5) tensor)."""
+        """Returns (J, dJ/dq0 per block, dJ/dq_inf as (nblocks, 6, 5) tensor).
+        checkpoint_every = k > 0 keeps the state every k steps and recomputes
+        one segment at a time in the reverse pass; k <= 0 keeps every step's
+        start and stage states from the forward pass (3 states per step in
+        device memory) and recomputes nothing."""
         torch = L._torch()
         t0 = time.perf_counter()
-        cur, checkpoints = self.forward_checkpointed(self.q0, dt, nsteps, checkpoint_every)
-        J, lam = self.objective(cur, kind, blocks)
         dqinf = torch.zeros(self.nb * 6 * 5, dtype=torch.float64, device="cuda")
-        t1 = time.perf_counter()
-        self.reverse(lam, dt, nsteps, checkpoints, checkpoint_every, dqinf)
+        if checkpoint_every <= 0:
+            cur = [q.clone() for q in self.q0]
+            starts, stages = [], []
+            for _ in range(nsteps):
+                starts.append([q.clone() for q in cur])
+                self._step(cur, dt)
+                q1s = [torch.empty_like(q) for q in cur]
+                q2s = [torch.empty_like(q) for q in cur]
+                self.plan.stage_states(q1s, q2s)
+                stages.append((q1s, q2s))
+            J, lam = self.objective(cur, kind, blocks)
+            t1 = time.perf_counter()
+            for n in reversed(range(nsteps)):
+                self.plan.step_vjp(starts[n], self.kxyz, self.vol, lam, dt, dqinf, batched=self.batched,
+                                   direct=self.direct, stages=stages[n])
+                starts[n] = stages[n] = None    # release as the reverse pass goes
+        else:
+            cur, checkpoints = self.forward_checkpointed(self.q0, dt, nsteps, checkpoint_every)
+            J, lam = self.objective(cur, kind, blocks)
+            t1 = time.perf_counter()
+            self.reverse(lam, dt, nsteps, checkpoints, checkpoint_every, dqinf)
         torch.cuda.synchronize()
         if timing is not None:
             timing["forward_with_checkpoints_s"] = t1 - t0

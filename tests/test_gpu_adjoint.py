@@ -183,6 +183,11 @@ def test_kept_stage_states_equal_recomputed():
         outs.append((lam[0].cpu().numpy(), dq.cpu().numpy()))
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    # every step's states kept from the forward pass (no recompute): same bits
+    J0, g0, dq0 = adj.gradient(dt, 3, OBJ_WALL_PRESSURE, checkpoint_every=0)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(g0[0].cpu().numpy(), outs[0][0])
+    np.testing.assert_array_equal(dq0.reshape(-1).cpu().numpy(), outs[0][1])
 
 
 def test_torch_autograd_function():

@@ -470,4 +470,28 @@ __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src,
   if (t < n) dst[t] = src[t];
 }
 
+// Up to COPY_JOBS copies in one launch (passed by value as the kernel
+// argument): job blockIdx.y copies n[job] doubles; 4 per thread.
+constexpr int COPY_JOBS = 64;
+struct CopyJobs {
+  double* dst[COPY_JOBS];
+  const double* src[COPY_JOBS];
+  int64_t n[COPY_JOBS];
+};
+
+__global__ void __launch_bounds__(256) k_copy_jobs(CopyJobs jobs) {
+  const int j = blockIdx.y;
+  const int64_t n = jobs.n[j];
+  double* __restrict__ d = jobs.dst[j];
+  const double* __restrict__ s = jobs.src[j];
+  for (int64_t t = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4; t < n; t += (int64_t)gridDim.x * 1024) {
+    if (t + 3 < n) {
+      const double a = s[t], b = s[t + 1], c = s[t + 2], e = s[t + 3];
+      d[t] = a; d[t + 1] = b; d[t + 2] = c; d[t + 3] = e;
+    } else {
+      for (int64_t u = t; u < n; ++u) d[u] = s[u];
+    }
+  }
+}
+
 }  // namespace famrdp

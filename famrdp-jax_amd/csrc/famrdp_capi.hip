@@ -182,6 +182,27 @@ int bind(famrdp_plan* p, double* const* q, const double* const* kxyz, const doub
   return FAMRDP_OK;
 }
 
+// Copies of one 5*N array per block (dst[b] <- src[b]) in as few launches
+// as the job table allows (COPY_JOBS blocks per launch).
+template <class Dst, class Src>
+int copy_blocks(famrdp_plan* p, hipStream_t st, Dst dst_of, Src src_of) {
+  for (int b0 = 0; b0 < p->nblocks; b0 += COPY_JOBS) {
+    const int nj = std::min(COPY_JOBS, p->nblocks - b0);
+    CopyJobs jobs{};
+    int64_t nmax = 0;
+    for (int j = 0; j < nj; ++j) {
+      jobs.dst[j] = dst_of(b0 + j);
+      jobs.src[j] = src_of(b0 + j);
+      jobs.n[j] = 5 * p->N[b0 + j];
+      nmax = std::max(nmax, jobs.n[j]);
+    }
+    const int64_t gx = std::min<int64_t>((nmax + 1023) / 1024, 65535);
+    k_copy_jobs<<<dim3((unsigned)gx, (unsigned)nj), 256, 0, st>>>(jobs);
+    LAUNCH_CHECK();
+  }
+  return FAMRDP_OK;
+}
+
 int launch_halo(famrdp_plan* p, hipStream_t st, const BlockDesc* table = nullptr) {
   if (p->nhalo == 0) return FAMRDP_OK;
   const BlockDesc* tb = table ? table : p->d_blocks;
@@ -724,30 +745,25 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
   int rc = bind(p, q_start, kxyz, vol, st);
   if (rc) return rc;
   // ---- q0' = BC(halo(q_start)) into qh (BC snapshots kept), then stages 1-2
-  for (int b = 0; b < p->nblocks; ++b) {
-    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].qh, q_start[b], 5 * p->N[b]);
-    LAUNCH_CHECK();
-  }
+  rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].qh; }, [&](int b) { return q_start[b]; });
+  if (rc) return rc;
   rc = launch_halo(p, st, p->d_blocks_adj);
   if (rc) return rc;
   rc = launch_bc(p, st, p->d_blocks_adj, true);
   if (rc) return rc;
   if (q1s) {
-    for (int b = 0; b < p->nblocks; ++b) {
-      k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].q1, q1s[b], 5 * p->N[b]);
-      LAUNCH_CHECK();
-      k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].q2, q2s[b], 5 * p->N[b]);
-      LAUNCH_CHECK();
-    }
+    rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].q1; }, [&](int b) { return q1s[b]; });
+    if (rc) return rc;
+    rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].q2; }, [&](int b) { return q2s[b]; });
+    if (rc) return rc;
   } else {
     rc = launch_stages(p, dt, flags, st, true, 2);
     if (rc) return rc;
   }
   // ---- reverse the three stages
-  for (int b = 0; b < p->nblocks; ++b) {
-    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(p->h_blocks_adj[b].la, lam[b], 5 * p->N[b]);
-    LAUNCH_CHECK();
-  }
+  rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].la; },
+                   [&](int b) { return static_cast<const double*>(lam[b]); });
+  if (rc) return rc;
   for (int stage = 3; stage >= 1; --stage) {
     k_vjp_faces<<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas.gamma, p->limiter, dt);
     LAUNCH_CHECK();
@@ -775,10 +791,9 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
     k_halo_vjp_apply<<<nblk(p->nhalo_src), 256, 0, st>>>(p->d_blocks_adj, p->d_halo_src, p->nhalo_src, p->d_halo_S);
     LAUNCH_CHECK();
   }
-  for (int b = 0; b < p->nblocks; ++b) {
-    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(lam[b], p->h_blocks_adj[b].l0, 5 * p->N[b]);
-    LAUNCH_CHECK();
-  }
+  rc = copy_blocks(p, st, [&](int b) { return lam[b]; },
+                   [&](int b) { return static_cast<const double*>(p->h_blocks_adj[b].l0); });
+  if (rc) return rc;
   return FAMRDP_OK;
 }
 
@@ -802,14 +817,13 @@ int famrdp_plan_stage_states(famrdp_plan* p, double* const* q1_out, double* cons
   if (!p || !q1_out || !q2_out) return fail(FAMRDP_E_INVALID, "stage_states: bad arguments");
   if (!p->has_workspace) return fail(FAMRDP_E_INVALID, "stage_states: plan has no stage workspace");
   hipStream_t st = S(stream);
-  for (int b = 0; b < p->nblocks; ++b) {
+  for (int b = 0; b < p->nblocks; ++b)
     if (!q1_out[b] || !q2_out[b]) return fail(FAMRDP_E_INVALID, "stage_states: NULL pointer for block " + std::to_string(b));
-    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(q1_out[b], p->h_blocks[b].q1, 5 * p->N[b]);
-    LAUNCH_CHECK();
-    k_copy<<<nblk(5 * p->N[b]), 256, 0, st>>>(q2_out[b], p->h_blocks[b].q2, 5 * p->N[b]);
-    LAUNCH_CHECK();
-  }
-  return FAMRDP_OK;
+  int rc = copy_blocks(p, st, [&](int b) { return q1_out[b]; },
+                       [&](int b) { return static_cast<const double*>(p->h_blocks[b].q1); });
+  if (rc) return rc;
+  return copy_blocks(p, st, [&](int b) { return q2_out[b]; },
+                     [&](int b) { return static_cast<const double*>(p->h_blocks[b].q2); });
 }
 
 int famrdp_objective(int32_t kind, const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj,

@@ -505,6 +505,8 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     const int32_t* d = &dims[3 * b];
     if (d[0] < 2 * ghost || d[1] < 2 * ghost || d[2] < 2 * ghost)
       return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": each padded size must be >= 2*ghost"));
+    if ((int64_t)d[0] * d[1] * d[2] > INT32_MAX)   // tile kernels index cells in 32 bits
+      return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": 2^31 or more cells"));
     p->N.push_back((int64_t)d[0] * d[1] * d[2]);
   }
   // workspace and block table
@@ -527,6 +529,7 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     bd.q1 = q1; bd.q2 = q2; bd.G = G;
     bd.ni = dims[3 * b]; bd.nj = dims[3 * b + 1]; bd.nk = dims[3 * b + 2];
     bd.N = p->N[b];
+    bd.wrap_mod = tile_wrap_mod(bd.ni, bd.nj, bd.nk);
     bd.mu0 = p->gas.mu0;
     bd.dt_scale = 1.0;
   }

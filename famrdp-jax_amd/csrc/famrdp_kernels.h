@@ -19,8 +19,8 @@ struct BlockDesc {
   double* q2;           // stage-2 result (workspace)
   double* G;            // viscous flux tensor (9, N) (workspace, direct path)
   int32_t ni, nj, nk;
-  int32_t pad_;
-  int64_t N;            // ni*nj*nk
+  int32_t wrap_mod;     // 1: tile kernels wrap coordinates with a modulo (set by plan_create)
+  int64_t N;            // ni*nj*nk (< 2^31 for blocks of a plan)
   double mu0;           // Sutherland mu0 of this block (ensembles of cases)
   double dt_scale;      // this block's time step = dt * dt_scale
   // adjoint workspace (null unless the plan was created for reverse sweeps)

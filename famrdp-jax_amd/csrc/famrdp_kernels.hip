@@ -651,10 +651,41 @@ struct FusedGeom {
   }
 };
 
-__device__ __forceinline__ int wrap_to(int i, int n) {
-  while (i < 0) i += n;
-  while (i >= n) i -= n;
-  return i;
+// Periodic (roll) wrap of tile coordinates into a block and the cell index.
+// Tiles start inside the block and the cells a tile kernel touches lie at
+// most 2 cells before and 22 cells past its origin, so in a block whose padded
+// sizes are all >= WRAP_ONCE_MIN a coordinate lies in [-n, 2n) and one
+// conditional add or subtract wraps it; written as unsigned minima it takes
+// two full-rate instructions and no branch.  With nj*nk < 2^24 the index then
+// comes from 24-bit multiplies.  Other blocks (BlockDesc::wrap_mod) take a
+// modulo; the choice is uniform per block.  Plans hold blocks of fewer than
+// 2^31 cells, so indices are 32-bit.
+constexpr int WRAP_ONCE_MIN = 40;
+
+__host__ __device__ inline int tile_wrap_mod(int ni, int nj, int nk) {
+  return (ni < WRAP_ONCE_MIN || nj < WRAP_ONCE_MIN || nk < WRAP_ONCE_MIN || (int64_t)nj * nk >= (1 << 24)) ? 1 : 0;
+}
+
+__device__ __forceinline__ int wrap_once(int i, int n) {   // i in [-n, 2n)
+  const unsigned a = min((unsigned)i, (unsigned)(i + n));
+  return (int)min(a, a - (unsigned)n);
+}
+
+__device__ __forceinline__ int wrap_mod(int i, int n) {
+  const int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// index of cell (i, j, k) of the block, each coordinate wrapped
+template <bool MOD>
+__device__ __forceinline__ int tile_cell(int i, int j, int k, int ni, int nj, int nk) {
+  if constexpr (MOD) {
+    return (wrap_mod(i, ni) * nj + wrap_mod(j, nj)) * nk + wrap_mod(k, nk);
+  } else {
+    return (int)(__umul24((unsigned)wrap_once(i, ni), (unsigned)(nj * nk)) +
+                 __umul24((unsigned)wrap_once(j, nj), (unsigned)nk)) +
+           wrap_once(k, nk);
+  }
 }
 
 // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so map
@@ -753,25 +784,69 @@ __device__ __forceinline__ void face_axis(const BlockDesc& b, int i0, int j0, in
   __syncthreads();
 }
 
-// Staging loads of one tile into registers: thread t's cells t, t + NT, ...
-// of the primitive slots.  Every thread issues the same number of loads
-// (indices past the last slot are clamped to it and the values unused), so
-// the compiler can wait for older loads without waiting for these.
+// Cell indices a thread of the fused kernel loads from: its staging slots
+// t, t + NT, ... (indices past the last slot clamped to it, the values
+// unused), the G cells beyond the tile it computes, and its own tile cell
+// (wrapped into the block when outside it).  Computed once per tile; the
+// wrap form is chosen per block by a uniform branch.
 template <int TI, int TJ, int TK>
-__device__ __forceinline__ void fused_stage_loads(const BlockDesc& b, const double* __restrict__ qin, int i0, int j0,
-                                                  int k0, double qq[][5]) {
+struct FusedCells {
   using T = FusedGeom<TI, TJ, TK>;
-  constexpr int NIT = (T::NPV + T::NT - 1) / T::NT;
-  const int t = threadIdx.x;
-  const int64_t si = (int64_t)b.nj * b.nk, sj = b.nk;
+  static constexpr int NIT = (T::NPV + T::NT - 1) / T::NT;
+  static constexpr int NGIT = (T::NG + T::NT - 1) / T::NT;
+  int stage[NIT];
+  int gh[NGIT > 1 ? NGIT - 1 : 1];
+  int c;
+};
+
+template <int TI, int TJ, int TK, bool MOD, bool ALL>
+__device__ __forceinline__ void fused_cells_fill(const BlockDesc& b, int i0, int j0, int k0, int t,
+                                                 FusedCells<TI, TJ, TK>& fc) {
+  using C = FusedCells<TI, TJ, TK>;
+  using T = typename C::T;
+  const int ni = b.ni, nj = b.nj, nk = b.nk;
 #pragma unroll
-  for (int it = 0; it < NIT; ++it) {
+  for (int it = 0; it < C::NIT; ++it) {
     const int h = t + it * T::NT;
     const int3 lc = T::pv_cell(h < T::NPV ? h : T::NPV - 1);
-    const int64_t cc = (int64_t)wrap_to(i0 + lc.x, b.ni) * si + (int64_t)wrap_to(j0 + lc.y, b.nj) * sj +
-                       wrap_to(k0 + lc.z, b.nk);
+    fc.stage[it] = tile_cell<MOD>(i0 + lc.x, j0 + lc.y, k0 + lc.z, ni, nj, nk);
+  }
+  if constexpr (!ALL) return;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * b.N + cc];
+  for (int it = 1; it < C::NGIT; ++it) {
+    const int h = t + it * T::NT;
+    const int3 lc = T::g_cell(h < T::NG ? h : T::NG - 1);
+    fc.gh[it - 1] = tile_cell<MOD>(i0 + lc.x, j0 + lc.y, k0 + lc.z, ni, nj, nk);
+  }
+  const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
+  fc.c = tile_cell<MOD>(i0 + x, j0 + y, k0 + z, ni, nj, nk);
+}
+
+// ALL = false: the staging slots only, for thread id t
+template <int TI, int TJ, int TK, bool ALL = true>
+__device__ __forceinline__ void fused_cells(const BlockDesc& b, int i0, int j0, int k0, int t,
+                                            FusedCells<TI, TJ, TK>& fc) {
+  if (b.wrap_mod) fused_cells_fill<TI, TJ, TK, true, ALL>(b, i0, j0, k0, t, fc);
+  else fused_cells_fill<TI, TJ, TK, false, ALL>(b, i0, j0, k0, t, fc);
+}
+
+// v, hidden from the optimiser: values computed from it are computed again
+// rather than kept in registers from an earlier, identical computation
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Staging loads of one tile into registers.  Every thread issues the same
+// number of loads, so the compiler can wait for older loads without waiting
+// for these.
+template <int TI, int TJ, int TK>
+__device__ __forceinline__ void fused_stage_loads(const BlockDesc& b, const double* __restrict__ qin,
+                                                  const FusedCells<TI, TJ, TK>& fc, double qq[][5]) {
+#pragma unroll
+  for (int it = 0; it < FusedCells<TI, TJ, TK>::NIT; ++it) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * b.N + fc.stage[it]];
   }
 }
 
@@ -786,9 +861,10 @@ __device__ __forceinline__ void fused_stage_loads(const BlockDesc& b, const doub
 template <int TI, int TJ, int TK, int LIM, bool RS>
 __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0, int j0, int k0,
                                                       const double* __restrict__ qin, const double* q0, double* qout,
-                                                      const Gas& gas, double dt, int stage, double qq[][5]) {
+                                                      const Gas& gas, double dt, int stage,
+                                                      const FusedCells<TI, TJ, TK>& fc, double qq[][5]) {
   using T = FusedGeom<TI, TJ, TK>;
-  constexpr int NIT = (T::NPV + T::NT - 1) / T::NT;
+  constexpr int NIT = FusedCells<TI, TJ, TK>::NIT;
   __shared__ double lds[RS ? T::NLDS_RS : T::NLDS];
   double* s_pv = lds;                                  // [5][NPV]
   double* s_T = lds + 5 * T::NPV;                      // [NPV]   temperature of the staged cells
@@ -797,7 +873,6 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
   const int t = threadIdx.x;
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
-  const int64_t si = (int64_t)nj * nk, sj = nk;
   const double g = gas.gamma;
   Gas gb = gas;
   gb.mu0 = b.mu0;
@@ -805,11 +880,8 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
   const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
   const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
   const bool own = gi < ni && gj < nj && gk < nk;
-  auto gidx_wrapped = [&](int lx, int ly, int lz) -> int64_t {
-    return (int64_t)wrap_to(i0 + lx, ni) * si + (int64_t)wrap_to(j0 + ly, nj) * sj + wrap_to(k0 + lz, nk);
-  };
   // the tile cell of this thread (wrapped into the block when outside it)
-  const int64_t c = gidx_wrapped(x, y, z);
+  const int c = fc.c;
 
   // ---- 1. primitives from the staged registers --------------------------------
 #pragma unroll
@@ -828,15 +900,12 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
   double kk[9];
 #pragma unroll
   for (int m = 0; m < 9; ++m) kk[m] = b.kxyz[m * N + c];
-  constexpr int NGIT = (T::NG + T::NT - 1) / T::NT;
+  constexpr int NGIT = FusedCells<TI, TJ, TK>::NGIT;
   double kh[NGIT > 1 ? NGIT - 1 : 1][9];
 #pragma unroll
   for (int it = 1; it < NGIT; ++it) {
-    const int h = t + it * T::NT;
-    const int3 lc = T::g_cell(h < T::NG ? h : T::NG - 1);
-    const int64_t cc = gidx_wrapped(lc.x, lc.y, lc.z);
 #pragma unroll
-    for (int m = 0; m < 9; ++m) kh[it - 1][m] = b.kxyz[m * N + cc];
+    for (int m = 0; m < 9; ++m) kh[it - 1][m] = b.kxyz[m * N + fc.gh[it - 1]];
   }
   __syncthreads();
 
@@ -882,7 +951,11 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
         for (int m = 0; m < 9; ++m) s_g[m * T::NG + h] = gsave[it][m];
       }
     }
-    fused_stage_loads<TI, TJ, TK>(b, qin, i0, j0, k0, qq);   // for the second staging
+    // indices of the second staging computed again (held through the G
+    // phase they would cost registers the three-per-CU budget lacks)
+    FusedCells<TI, TJ, TK> fc2;
+    fused_cells<TI, TJ, TK, false>(b, i0, j0, k0, opaque(t), fc2);
+    fused_stage_loads<TI, TJ, TK>(b, qin, fc2, qq);
     __syncthreads();
   }
   double vis[4] = {0.0, 0.0, 0.0, 0.0};
@@ -951,10 +1024,11 @@ template <int TI, int TJ, int TK, int LIM, bool RS>
 __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int j0, int k0,
                                                  const double* __restrict__ qin, const double* q0, double* qout,
                                                  const Gas& gas, double dt, int stage) {
-  using T = FusedGeom<TI, TJ, TK>;
-  double qq[(T::NPV + T::NT - 1) / T::NT][5];
-  fused_stage_loads<TI, TJ, TK>(b, qin, i0, j0, k0, qq);
-  stage_tile_fused_regs<TI, TJ, TK, LIM, RS>(b, i0, j0, k0, qin, q0, qout, gas, dt, stage, qq);
+  FusedCells<TI, TJ, TK> fc;
+  fused_cells<TI, TJ, TK>(b, i0, j0, k0, (int)threadIdx.x, fc);
+  double qq[FusedCells<TI, TJ, TK>::NIT][5];
+  fused_stage_loads<TI, TJ, TK>(b, qin, fc, qq);
+  stage_tile_fused_regs<TI, TJ, TK, LIM, RS>(b, i0, j0, k0, qin, q0, qout, gas, dt, stage, fc, qq);
 }
 
 // The re-staging form targets three workgroups per CU: at most 168 VGPRs.
@@ -1066,16 +1140,24 @@ __device__ __forceinline__ void visc_tile(const BlockDesc& b, int i0, int j0, in
   gb.mu0 = b.mu0;
   {
     constexpr int NIT = (T::NG + T::NT - 1) / T::NT;
+    int cc[NIT];
+    auto cells = [&](auto mod) {
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int h = t + it * T::NT;
+        const int3 lc = T::g_cell(h < T::NG ? h : T::NG - 1);
+        cc[it] = tile_cell<decltype(mod)::value>(i0 + lc.x, j0 + lc.y, k0 + lc.z, ni, nj, nk);
+      }
+    };
+    if (b.wrap_mod) cells(std::true_type{});
+    else cells(std::false_type{});
     double qq[NIT][5];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int h = t + it * T::NT;
       if (h < T::NG) {
-        const int3 lc = T::g_cell(h);
-        const int64_t cc =
-            (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * N + cc];
+        for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * N + cc[it]];
       }
     }
 #pragma unroll
@@ -1135,22 +1217,32 @@ __device__ __forceinline__ void stage_tile_split(const BlockDesc& b, int i0, int
   const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
   const int gi = i0 + x, gj = j0 + y, gk = k0 + z;
   const bool own = t < T::NT && gi < ni && gj < nj && gk < nk;
-  // the thread's cell, wrapped into the block when outside it (a valid address)
-  const int64_t c = (int64_t)wrap_to(gi, ni) * si + (int64_t)wrap_to(gj, nj) * sj + wrap_to(gk, nk);
+  // the thread's cell (wrapped into the block when outside it: a valid
+  // address) and its staging cells
+  constexpr int NIT = (T::NP + NTH - 1) / NTH;
+  int c, cc[NIT];
+  auto cells = [&](auto mod) {
+    constexpr bool M = decltype(mod)::value;
+    c = tile_cell<M>(gi, gj, gk, ni, nj, nk);
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int h = t + it * NTH;
+      const int3 lc = T::p_cell(h < T::NP ? h : T::NP - 1);
+      cc[it] = tile_cell<M>(i0 + lc.x, j0 + lc.y, k0 + lc.z, ni, nj, nk);
+    }
+  };
+  if (b.wrap_mod) cells(std::true_type{});
+  else cells(std::false_type{});
 
   // ---- 1. primitives of the tile and the two-cell cross halo ------------------
   {
-    constexpr int NIT = (T::NP + NTH - 1) / NTH;
     double qq[NIT][5];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int h = t + it * NTH;
       if (h < T::NP) {
-        const int3 lc = T::p_cell(h);
-        const int64_t cc =
-            (int64_t)wrap_to(i0 + lc.x, ni) * si + (int64_t)wrap_to(j0 + lc.y, nj) * sj + wrap_to(k0 + lc.z, nk);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * N + cc];
+        for (int v = 0; v < 5; ++v) qq[it][v] = qin[v * N + cc[it]];
       }
     }
 #pragma unroll

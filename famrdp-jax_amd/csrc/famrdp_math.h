@@ -286,17 +286,16 @@ DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3],
   const double unL = uL * n[0] + vL * n[1] + wL * n[2];
   const double unR = uR * n[0] + vR * n[1] + wR * n[2];
   const double w1 = al1 * a1, w5 = al5 * a5, w2 = al2 * a23, w2r = al2 * rhoh;
-  // F = 1/2 (FL + FR) - 1/2 (w1 R1 + w5 R5 + al2 (a23 R2 + dvt))
-  const double cn0 = ch * n[0], cn1 = ch * n[1], cn2 = ch * n[2], cun = ch * unh;
-  F[0] = 0.5 * (rhoL * unL + rhoR * unR) - 0.5 * (w1 + w5 + w2);
-  F[1] = 0.5 * ((rhoL * uL * unL + pL * n[0]) + (rhoR * uR * unR + pR * n[0])) -
-         0.5 * (w1 * (uh - cn0) + w5 * (uh + cn0) + (w2 * uh + w2r * tx));
-  F[2] = 0.5 * ((rhoL * vL * unL + pL * n[1]) + (rhoR * vR * unR + pR * n[1])) -
-         0.5 * (w1 * (vh - cn1) + w5 * (vh + cn1) + (w2 * vh + w2r * ty));
-  F[3] = 0.5 * ((rhoL * wL * unL + pL * n[2]) + (rhoR * wR * unR + pR * n[2])) -
-         0.5 * (w1 * (wh - cn2) + w5 * (wh + cn2) + (w2 * wh + w2r * tz));
-  F[4] = 0.5 * ((EL + pL) * unL + (ER + pR) * unR) -
-         0.5 * (w1 * (Hh - cun) + w5 * (Hh + cun) + (w2 * (0.5 * q2) + w2r * (uh * tx + vh * ty + wh * tz)));
+  // F = 1/2 (FL + FR) - 1/2 (w1 R1 + w5 R5 + al2 (a23 R2 + dvt)), with the
+  // acoustic pair gathered: w1 (u - c n) + w5 (u + c n) = (w1 + w5) u + c (w5 - w1) n
+  const double mL = rhoL * unL, mR = rhoR * unR;        // mass fluxes
+  const double sw = w1 + w5, dw = ch * (w5 - w1), sa = sw + w2;
+  F[0] = 0.5 * ((mL + mR) - sa);
+  F[1] = 0.5 * (((mL * uL + pL * n[0]) + (mR * uR + pR * n[0])) - (uh * sa + n[0] * dw + w2r * tx));
+  F[2] = 0.5 * (((mL * vL + pL * n[1]) + (mR * vR + pR * n[1])) - (vh * sa + n[1] * dw + w2r * ty));
+  F[3] = 0.5 * (((mL * wL + pL * n[2]) + (mR * wR + pR * n[2])) - (wh * sa + n[2] * dw + w2r * tz));
+  F[4] = 0.5 * (((EL + pL) * unL + (ER + pR) * unR) -
+                (Hh * sw + unh * dw + w2 * (0.5 * q2) + w2r * (uh * tx + vh * ty + wh * tz)));
 }
 
 // One face from the primitives of cells a-1, a, a+1, a+2 and kxyz[d, :] at

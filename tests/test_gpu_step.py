@@ -179,6 +179,32 @@ def test_tile_variants_bitwise_identical_and_autotune():
         st.plan.set_tile(0)
 
 
+def test_tile_variants_single_wrap_blocks_vs_oracle():
+    """Blocks whose padded sizes are all >= 40 take the tile kernels' single
+    conditional wrap (smaller ones a modulo): the C2 grid at 40^3 (44^3
+    padded, a partial last tile for every tile shape, periodic wrap at both
+    ends of each axis), one step per variant against the oracle, fused
+    variants bitwise among themselves."""
+    print("case: C2 grid 40^3, every tile variant vs oracle, 1 step", flush=True)
+    state, cfg, dt = _c2_small(40)
+    ref = _ostep([state.blocks[0].q], state, cfg, dt)[0]
+    fused = None
+    for v in range(_lib.NUM_TILE_VARIANTS):
+        st = Stepper(to_dev(state), cfg)
+        st.plan.set_tile(v)
+        st.advance(dt, batched=(v % 2 == 1))
+        torch.cuda.synchronize()
+        out = st.qs[0].cpu().numpy()
+        err = rel_linf(out, ref)
+        print(f"  variant {v}: rel Linf {err:.3e}", flush=True)
+        assert err <= STEP_TOL
+        if v in _lib.FUSED_VARIANTS:
+            if fused is None:
+                fused = out
+            else:
+                np.testing.assert_array_equal(out, fused)
+
+
 def test_c5_ensemble_per_block_mu0_and_dt():
     """Independent periodic cases with their own Mach, Reynolds number (mu0)
     and CFL time step, advanced together by one batched plan

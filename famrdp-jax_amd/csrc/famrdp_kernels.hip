@@ -1031,9 +1031,10 @@ __device__ __forceinline__ void stage_tile_fused(const BlockDesc& b, int i0, int
   stage_tile_fused_regs<TI, TJ, TK, LIM, RS>(b, i0, j0, k0, qin, q0, qout, gas, dt, stage, fc, qq);
 }
 
-// The re-staging form targets three workgroups per CU: at most 168 VGPRs.
-template <int TI, int TJ, int TK, int LIM, bool RS>
-__global__ void __launch_bounds__(TI * TJ * TK) __attribute__((amdgpu_waves_per_eu(RS ? 3 : 1)))
+// WPE: waves per SIMD the register allocation aims at (3 for the re-staging
+// form at 256 threads: three workgroups per CU, at most 168 VGPRs).
+template <int TI, int TJ, int TK, int LIM, bool RS, int WPE>
+__global__ void __launch_bounds__(TI * TJ * TK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_stage_fused(BlockDesc b, int stage, Gas gas, double dt, int ntj, int ntk) {
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
@@ -1042,8 +1043,8 @@ k_stage_fused(BlockDesc b, int stage, Gas gas, double dt, int ntj, int ntk) {
   stage_tile_fused<TI, TJ, TK, LIM, RS>(b, ti * TI, tj * TJ, tk * TK, qin, b.q, qout, gas, dt, stage);
 }
 
-template <int TI, int TJ, int TK, int LIM, bool RS>
-__global__ void __launch_bounds__(TI * TJ * TK) __attribute__((amdgpu_waves_per_eu(RS ? 3 : 1)))
+template <int TI, int TJ, int TK, int LIM, bool RS, int WPE>
+__global__ void __launch_bounds__(TI * TJ * TK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_stage_fused_batched(const BlockDesc* __restrict__ blocks, const Tile3* __restrict__ tiles, int stage, Gas gas,
                       double dt) {
   const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
@@ -1351,35 +1352,37 @@ __global__ void __launch_bounds__(NTH) k_stage_split_batched(const BlockDesc* __
   stage_tile_split<TI, TJ, TK, NTH, LIM>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, dt, stage);
 }
 
-// Stage-kernel variants the autotuner chooses from (ids 0-4 and 10-12
-// fused, 5-9 split pair).  Fused kernel: (id, TI, TJ, TK, re-staging), one
-// thread per cell.  Split pair: (id, TI, TJ, TK,
-// threads of the stage kernel): with one thread per cell an axis's faces
-// take two passes, with a thread per face (NTH1) one.
+// Stage-kernel variants the autotuner chooses from (ids 0-4 and 10-13
+// fused, 5-9 split pair).  Fused kernel: (id, TI, TJ, TK, re-staging, waves
+// per SIMD for the register allocation), one thread per cell.  Split pair:
+// (id, TI, TJ, TK, threads of the stage kernel): with one thread per cell an
+// axis's faces take two passes, with a thread per face (NTH1) one.
 #define FAMRDP_TILE_VARIANTS(X) \
-  X(0, 8, 4, 8, false)          \
-  X(1, 4, 8, 16, false)         \
-  X(2, 8, 4, 16, false)         \
-  X(3, 4, 8, 8, false)          \
-  X(4, 4, 8, 8, true)           \
-  X(10, 8, 4, 8, true)          \
-  X(11, 8, 8, 4, true)          \
-  X(12, 4, 4, 16, true)
+  X(0, 8, 4, 8, false, 1)       \
+  X(1, 4, 8, 16, false, 1)      \
+  X(2, 8, 4, 16, false, 1)      \
+  X(3, 4, 8, 8, false, 1)       \
+  X(4, 4, 8, 8, true, 3)        \
+  X(10, 8, 4, 8, true, 3)       \
+  X(11, 8, 8, 4, true, 3)       \
+  X(12, 4, 4, 16, true, 3)      \
+  X(13, 8, 8, 8, true, 4)
 #define FAMRDP_SPLIT_VARIANTS(X)  \
   X(5, 4, 4, 16, 256)             \
   X(6, 4, 8, 8, 256)              \
   X(7, 8, 8, 8, 512)              \
   X(8, 2, 8, 16, 256)             \
   X(9, 4, 8, 8, 320)
-constexpr int NUM_FUSED_VARIANTS = 5;   // ids 0-4 and 10-12 are the fused kernel
-constexpr int NUM_TILE_VARIANTS = 13;
+constexpr int NUM_TILE_VARIANTS = 14;
 
 inline void tile_dims(int v, int& ti, int& tj, int& tk) {
   switch (v) {
 #define FAMRDP_TD5(id, a, b, c, n) case id: ti = a; tj = b; tk = c; return;
-    FAMRDP_TILE_VARIANTS(FAMRDP_TD5)
+#define FAMRDP_TD6(id, a, b, c, rs, w) case id: ti = a; tj = b; tk = c; return;
+    FAMRDP_TILE_VARIANTS(FAMRDP_TD6)
     FAMRDP_SPLIT_VARIANTS(FAMRDP_TD5)
 #undef FAMRDP_TD5
+#undef FAMRDP_TD6
     default: ti = 4; tj = 4; tk = 16;
   }
 }
@@ -1394,8 +1397,8 @@ inline void launch_stage_lim(int v, const BlockDesc& d, int stage, const Gas& ga
   const int nti = (d.ni + TI - 1) / TI, ntj = (d.nj + TJ - 1) / TJ, ntk = (d.nk + TK - 1) / TK;
   const int nt = nti * ntj * ntk;
   switch (v) {
-#define FAMRDP_TL(id, a, b, c, rs) \
-  case id: k_stage_fused<a, b, c, LIM, rs><<<nt, a * b * c, 0, st>>>(d, stage, gas, dt, ntj, ntk); return;
+#define FAMRDP_TL(id, a, b, c, rs, w) \
+  case id: k_stage_fused<a, b, c, LIM, rs, w><<<nt, a * b * c, 0, st>>>(d, stage, gas, dt, ntj, ntk); return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TL)
 #undef FAMRDP_TL
 #define FAMRDP_SL(id, a, b, c, n)                                                       \
@@ -1420,9 +1423,9 @@ template <int LIM>
 inline void launch_stage_batched_lim(int v, const BlockDesc* blocks, const Tile3* tiles, int ntiles, int stage,
                                      const Gas& gas, double dt, int ncu, hipStream_t st) {
   switch (v) {
-#define FAMRDP_TB(id, a, b, c, rs)                                                                       \
-  case id:                                                                                               \
-    k_stage_fused_batched<a, b, c, LIM, rs><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, dt); \
+#define FAMRDP_TB(id, a, b, c, rs, w)                                                                       \
+  case id:                                                                                                   \
+    k_stage_fused_batched<a, b, c, LIM, rs, w><<<ntiles, a * b * c, 0, st>>>(blocks, tiles, stage, gas, dt); \
     return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TB)
 #undef FAMRDP_TB

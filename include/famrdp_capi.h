@@ -134,14 +134,14 @@ int famrdp_step(famrdp_plan* plan, double* const* q, const double* const* kxyz, 
 int famrdp_step_graph(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                       double dt, int32_t flags, int32_t nsteps, void* stream);
 /* Stage-kernel selection (no counterpart in the reference).  Variants 0-4
- * and 10-12 are tile shapes of the fused stage kernel (4 and 10-12 stage the
+ * and 10-13 are tile shapes of the fused stage kernel (4 and 10-13 stage the
  * primitives twice to need less LDS), 5-9 tile shapes of the split pair
  * (viscous tensor kernel + stage kernel).  famrdp_plan_autotune times `reps`
  * stage-1 launches of every variant for these arrays (stage 1 writes only the
  * plan's workspace, so the state is not modified), writes the per-variant ms
  * to ms_out (may be NULL; FAMRDP_NUM_TILE_VARIANTS entries) and keeps the
  * fastest. */
-#define FAMRDP_NUM_TILE_VARIANTS 13
+#define FAMRDP_NUM_TILE_VARIANTS 14
 int famrdp_plan_autotune(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                          double dt, int32_t flags, int32_t reps, double* ms_out, void* stream);
 int famrdp_plan_set_tile(famrdp_plan* plan, int32_t variant);

@@ -56,8 +56,9 @@ FD void cons_to_prim_vjp(const double q[5], double g, const double lp[5], double
 }
 
 // Limiter reverse: s = lim(A, B) (physics/reconstruct/muscl2.py:5-12).
-FD void limiter_vjp(int lim, double A, double B, double ls, double& lA, double& lB) {
-  if (lim == LIM_VANLEER) {
+template <int LIM>
+FD void limiter_vjp_t(double A, double B, double ls, double& lA, double& lB) {
+  if constexpr (LIM == LIM_VANLEER) {
     if (A * B > 0.0) {
       const double ab = A + B;
       const bool use = fabs(ab) > 1e-30;
@@ -69,16 +70,23 @@ FD void limiter_vjp(int lim, double A, double B, double ls, double& lA, double& 
     } else {
       lA = 0.0; lB = 0.0;
     }
-    return;
+  } else {
+    // minmod: s = S min(|A|, |B|), S = sign(A) max(sign(A) sign(B), 0), dS = 0.
+    // S is nonzero only when A and B have one strict sign, and then
+    // S |.|'(A) = S |.|'(B) = 1: the cotangent goes to the smaller magnitude
+    // (split 1/2 on a tie).
+    const bool same = (A > 0.0 && B > 0.0) || (A < 0.0 && B < 0.0);
+    const double xa = fabs(A), xb = fabs(B);
+    const double wa = xa < xb ? 1.0 : (xa == xb ? 0.5 : 0.0);
+    const double wb = xb < xa ? 1.0 : (xa == xb ? 0.5 : 0.0);
+    lA = same ? ls * wa : 0.0;
+    lB = same ? ls * wb : 0.0;
   }
-  // minmod: s = S * min(|A|, |B|), S = sign(A) max(sign(A) sign(B), 0), dS = 0
-  const double sa = sgn(A);
-  const double S = sa * fmax(sa * sgn(B), 0.0);
-  const double xa = fabs(A), xb = fabs(B);
-  const double wa = xa < xb ? 1.0 : (xa == xb ? 0.5 : 0.0);
-  const double wb = xb < xa ? 1.0 : (xa == xb ? 0.5 : 0.0);
-  lA = ls * S * wa * dabs(A);
-  lB = ls * S * wb * dabs(B);
+}
+
+FD void limiter_vjp(int lim, double A, double B, double ls, double& lA, double& lB) {
+  if (lim == LIM_VANLEER) limiter_vjp_t<LIM_VANLEER>(A, B, ls, lA, lB);
+  else limiter_vjp_t<LIM_MINMOD>(A, B, ls, lA, lB);
 }
 
 // Forward Roe flux on primitives (identical arithmetic to roe_flux_prim) with
@@ -272,11 +280,12 @@ FD void roe_flux_prim_vjp(const double pl[5], const double pr[5], const double n
 // F * |n| for the face between cells a and a+1 whose left/right states come
 // from cells a-1..a+2, return the cotangents of the reconstructed states and
 // of the two limited slopes:  pL = pv1 + s1/2, pR = pv2 - s2/2.
-FD void face_states(const double pv0[5], const double pv1[5], const double pv2[5], const double pv3[5], int lim,
+template <int LIM>
+FD void face_states(const double pv0[5], const double pv1[5], const double pv2[5], const double pv3[5],
                     double pL[5], double pR[5]) {
   for (int v = 0; v < 5; ++v) {
-    const double sL = limit(lim, pv1[v] - pv0[v], pv2[v] - pv1[v]);
-    const double sR = limit(lim, pv2[v] - pv1[v], pv3[v] - pv2[v]);
+    const double sL = limit(LIM, pv1[v] - pv0[v], pv2[v] - pv1[v]);
+    const double sR = limit(LIM, pv2[v] - pv1[v], pv3[v] - pv2[v]);
     pL[v] = pv1[v] + 0.5 * sL;
     pR[v] = pv2[v] - 0.5 * sR;
   }

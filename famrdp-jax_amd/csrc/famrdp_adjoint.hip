@@ -23,9 +23,9 @@ namespace famrdp {
 // ---------------------------------------------------------------------------
 // inviscid faces (rhs/invscd.py:53-131 reversed)
 // ---------------------------------------------------------------------------
+template <int LIM>
 __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double* __restrict__ qs,
-                                               const double* __restrict__ lin, double c_mu, int64_t a, double g,
-                                               int lim) {
+                                               const double* __restrict__ lin, double c_mu, int64_t a, double g) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
   const int k = (int)(a % nk), j = (int)((a / nk) % nj), i = (int)(a / ((int64_t)nk * nj));
@@ -56,7 +56,7 @@ __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double*
         cons_to_prim_fast(qq, g, pv[s]);
       }
       double pL[5], pR[5];
-      face_states(pv[0], pv[1], pv[2], pv[3], lim, pL, pR);
+      face_states<LIM>(pv[0], pv[1], pv[2], pv[3], pL, pR);
       double k0[3], k1[3];
 #pragma unroll
       for (int m = 0; m < 3; ++m) { k0[m] = b.kxyz[(d * 3 + m) * N + a]; k1[m] = b.kxyz[(d * 3 + m) * N + a1]; }
@@ -129,8 +129,9 @@ __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* 
 //   mode 1: l0 += c0 lin, lout = c_next lin + g     (stage 2)
 //   mode 2: l0 += lin + g                           (stage 1)
 // ---------------------------------------------------------------------------
+template <int LIM>
 __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double* __restrict__ qs,
-                                                const double* lin, double* lout, int64_t y, const Gas& gas, int lim,
+                                                const double* lin, double* lout, int64_t y, const Gas& gas,
                                                 double c_next, double c0, int mode) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
@@ -168,7 +169,7 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
         const double ls = 0.5 * fsd[v * N + xc] - 0.5 * fsd[(5 + v) * N + xc - sd];
         const double A = pv[o + 2][v] - pv[o + 1][v], B = pv[o + 3][v] - pv[o + 2][v];
         double lA, lB;
-        limiter_vjp(lim, A, B, ls, lA, lB);
+        limiter_vjp_t<LIM>(A, B, ls, lA, lB);
         if (o == 0) lpv[v] += lA - lB;
         else if (o == 1) lpv[v] -= lA;
         else lpv[v] += lB;
@@ -221,8 +222,9 @@ __device__ __forceinline__ double stage_cmu(int stage, double dt) {
   return stage == 3 ? (2.0 / 3.0) * dt : (stage == 2 ? 0.25 * dt : dt);
 }
 
+template <int LIM>
 __global__ void __launch_bounds__(256) k_vjp_faces(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
-                                                   int stage, double g, int lim, double dt) {
+                                                   int stage, double g, double dt) {
   const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
   const BlockDesc& b = blocks[tl.block];
   const int64_t c = (int64_t)tl.start + threadIdx.x;
@@ -230,7 +232,7 @@ __global__ void __launch_bounds__(256) k_vjp_faces(const BlockDesc* __restrict__
   const double *qs, *lin;
   double* lout;
   vjp_bufs(b, stage, qs, lin, lout);
-  cell_vjp_faces(b, qs, lin, stage_cmu(stage, dt * b.dt_scale), c, g, lim);
+  cell_vjp_faces<LIM>(b, qs, lin, stage_cmu(stage, dt * b.dt_scale), c, g);
 }
 
 // at most 128 VGPRs: these memory-bound gathers want four waves per SIMD
@@ -246,8 +248,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   cell_vjp_visc(b, qs, lin, stage_cmu(stage, dt * b.dt_scale), c, gas);
 }
 
+template <int LIM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vjp_cell(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
-                                                  int stage, Gas gas, int lim) {
+                                                  int stage, Gas gas) {
   const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
   const BlockDesc& b = blocks[tl.block];
   const int64_t c = (int64_t)tl.start + threadIdx.x;
@@ -257,7 +260,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   vjp_bufs(b, stage, qs, lin, lout);
   const double c_next = stage == 3 ? 2.0 / 3.0 : 0.25;
   const double c0 = stage == 3 ? 1.0 / 3.0 : 0.75;
-  cell_vjp_gather(b, qs, lin, lout, c, gas, lim, c_next, c0, stage == 3 ? 0 : (stage == 2 ? 1 : 2));
+  cell_vjp_gather<LIM>(b, qs, lin, lout, c, gas, c_next, c0, stage == 3 ? 0 : (stage == 2 ? 1 : 2));
 }
 
 // ---------------------------------------------------------------------------

@@ -768,11 +768,17 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
                    [&](int b) { return static_cast<const double*>(lam[b]); });
   if (rc) return rc;
   for (int stage = 3; stage >= 1; --stage) {
-    k_vjp_faces<<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas.gamma, p->limiter, dt);
+    if (p->limiter == LIM_VANLEER)
+      k_vjp_faces<LIM_VANLEER><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas.gamma, dt);
+    else
+      k_vjp_faces<LIM_MINMOD><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas.gamma, dt);
     LAUNCH_CHECK();
     k_vjp_visc<<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, dt);
     LAUNCH_CHECK();
-    k_vjp_cell<<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, p->limiter);
+    if (p->limiter == LIM_VANLEER)
+      k_vjp_cell<LIM_VANLEER><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas);
+    else
+      k_vjp_cell<LIM_MINMOD><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas);
     LAUNCH_CHECK();
   }
   // ---- BCs reversed (Face order backwards), then the halo copy

@@ -243,18 +243,24 @@ def main():
     c2 = bench_c2(args.n, args.steps, args.warmup, tile=args.tile)
     # HBM bytes per stage launch of the chosen variant, measured with rocprofv3
     # counters for this library version and size (profiles/stage_kernel_traffic.json)
-    traffic = None
+    # and the FP64 operations it executes (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64, FMA counted twice)
+    traffic = flops = None
     if os.path.exists(TRAFFIC_FILE):
         try:
             t = json.load(open(TRAFFIC_FILE))
             if t.get("version") == _lib.lib().famrdp_version().decode() and t.get("n") == args.n:
                 traffic = t.get("hbm_bytes_per_stage", {}).get(str(c2["variant"]))
+                flops = t.get("fp64_flop_per_stage", {}).get(str(c2["variant"]))
         except Exception:
-            traffic = None
+            traffic = flops = None
     roofline = dict(bound="hbm", achieved=round(c2["achieved_gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(c2["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
                     kernel="RK3 stage (" + _lib.lib().famrdp_version().decode() + ")",
                     bytes_per_launch=c2["stage_bytes"], avg_launch_ms=round(c2["avg_stage_ms"], 4))
+    if flops:
+        tf = flops / (c2["avg_stage_ms"] * 1e-3) / 1e12
+        roofline.update(fp64_flop_per_launch=flops, fp64_tflops=round(tf, 2), fp64_peak_tflops=FP64_PEAK_TFLOPS,
+                        fp64_frac=round(tf / FP64_PEAK_TFLOPS, 4))
     out = dict(metric=METRIC, value=round(c2["value"], 1), unit="cell-updates/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(c2["ms_per_step"], 4), higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="f64", data="synthetic",

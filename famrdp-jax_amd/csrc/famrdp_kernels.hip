@@ -570,12 +570,24 @@ __global__ void __launch_bounds__(256) k_cfl_final(const double* __restrict__ pa
 // ===========================================================================
 namespace famrdp {
 
+// Periodic (roll) wrap of tile coordinates into a block and the cell index.
+// Tiles start inside the block (origin <= n - 1) and the cells a tile kernel
+// touches lie at most 2 cells before and T + 1 past its origin along an axis
+// of tile extent T, so in a block whose padded sizes are all >= WRAP_ONCE_MIN
+// > T a coordinate lies in [-n, 2n) and one conditional add or subtract wraps
+// it; written as unsigned minima it takes two full-rate instructions and no
+// branch.  With nj*nk < 2^24 the index then comes from 24-bit multiplies.
+// Other blocks (BlockDesc::wrap_mod) take a modulo; the choice is uniform per
+// block.  Plans hold blocks of fewer than 2^31 cells, so indices are 32-bit.
+constexpr int WRAP_ONCE_MIN = 18;
+
 struct Tile3 {
   int32_t block, i0, j0, k0;
 };
 
 template <int TI, int TJ, int TK>
 struct FusedGeom {
+  static_assert(TI < WRAP_ONCE_MIN && TJ < WRAP_ONCE_MIN && TK < WRAP_ONCE_MIN, "tile longer than the single-wrap bound");
   static constexpr int NT = TI * TJ * TK;
   static constexpr int B0J = TJ + 2, B0K = TK + 2;
   static constexpr int NR0 = (TI + 2) * B0J * B0K;      // box with a one-cell halo
@@ -651,16 +663,6 @@ struct FusedGeom {
   }
 };
 
-// Periodic (roll) wrap of tile coordinates into a block and the cell index.
-// Tiles start inside the block and the cells a tile kernel touches lie at
-// most 2 cells before and 22 cells past its origin, so in a block whose padded
-// sizes are all >= WRAP_ONCE_MIN a coordinate lies in [-n, 2n) and one
-// conditional add or subtract wraps it; written as unsigned minima it takes
-// two full-rate instructions and no branch.  With nj*nk < 2^24 the index then
-// comes from 24-bit multiplies.  Other blocks (BlockDesc::wrap_mod) take a
-// modulo; the choice is uniform per block.  Plans hold blocks of fewer than
-// 2^31 cells, so indices are 32-bit.
-constexpr int WRAP_ONCE_MIN = 40;
 
 __host__ __device__ inline int tile_wrap_mod(int ni, int nj, int nk) {
   return (ni < WRAP_ONCE_MIN || nj < WRAP_ONCE_MIN || nk < WRAP_ONCE_MIN || (int64_t)nj * nk >= (1 << 24)) ? 1 : 0;
@@ -1071,6 +1073,7 @@ k_stage_fused_batched(const BlockDesc* __restrict__ blocks, const Tile3* __restr
 // ===========================================================================
 template <int TI, int TJ, int TK>
 struct SplitGeom {
+  static_assert(TI < WRAP_ONCE_MIN && TJ < WRAP_ONCE_MIN && TK < WRAP_ONCE_MIN, "tile longer than the single-wrap bound");
   static constexpr int NT = TI * TJ * TK;
   static constexpr int A0 = TJ * TK, A1 = TI * TK, A2 = TI * TJ;
   static constexpr int NP = NT + 4 * (A0 + A1 + A2);    // tile + two halo layers per side, per axis

@@ -179,14 +179,17 @@ def test_tile_variants_bitwise_identical_and_autotune():
         st.plan.set_tile(0)
 
 
-def test_tile_variants_single_wrap_blocks_vs_oracle():
-    """Blocks whose padded sizes are all >= 40 take the tile kernels' single
-    conditional wrap (smaller ones a modulo): the C2 grid at 40^3 (44^3
-    padded, a partial last tile for every tile shape, periodic wrap at both
-    ends of each axis), one step per variant against the oracle, fused
-    variants bitwise among themselves."""
-    print("case: C2 grid 40^3, every tile variant vs oracle, 1 step", flush=True)
-    state, cfg, dt = _c2_small(40)
+@pytest.mark.parametrize("n", [13, 14, 40])
+def test_tile_variants_single_wrap_blocks_vs_oracle(n):
+    """Blocks whose padded sizes are all >= 18 take the tile kernels' single
+    conditional wrap, smaller ones a modulo: the C2 grid at 13^3 (17^3
+    padded: modulo), 14^3 (18^3: the smallest single-wrap size, where a
+    16-cell tile's halo reaches 15 cells past the block end) and 40^3 (44^3:
+    a partial last tile for every tile shape), periodic wrap at both ends of
+    each axis; one step per variant against the oracle, fused variants
+    bitwise among themselves."""
+    print(f"case: C2 grid {n}^3, every tile variant vs oracle, 1 step", flush=True)
+    state, cfg, dt = _c2_small(n)
     ref = _ostep([state.blocks[0].q], state, cfg, dt)[0]
     fused = None
     for v in range(_lib.NUM_TILE_VARIANTS):

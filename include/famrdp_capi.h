@@ -109,9 +109,10 @@ int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, do
 
 /* ---- plan: one multi-block configuration (Config.topology + gas + scheme) - */
 
-/* dims: nblocks x {ni, nj, nk} (padded).  cuts/bcs as above.  Builds the
- * halo gather table (final writer per ghost cell, pre-exchange sources), the
- * per-face BC tables, the tile table and the stage workspace.
+/* dims: nblocks x {ni, nj, nk} (padded, each >= 2*ghost; a block of 2^31 or
+ * more cells is refused with FAMRDP_E_INVALID).  cuts/bcs as above.  Builds
+ * the halo gather table (final writer per ghost cell, pre-exchange sources),
+ * the per-face BC tables, the tile table and the stage workspace.
  * Replaces famrdp_jax/solver/step.py:18-56's per-call setup. */
 #define FAMRDP_PLAN_NO_WORKSPACE 1  /* halo / BC only: no stage buffers (famrdp_step is refused) */
 #define FAMRDP_PLAN_ADJOINT 2       /* also size the reverse-sweep workspace (famrdp_step_vjp) */

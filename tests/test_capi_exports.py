@@ -44,6 +44,22 @@ def test_variant_count_matches_header():
     assert int(re.search(r"#define FAMRDP_NUM_TILE_VARIANTS (\d+)", hdr).group(1)) == _lib.NUM_TILE_VARIANTS
 
 
+def test_plan_create_rejects_bad_block_sizes_before_allocating():
+    """famrdp_plan_create validates every block before any HIP call: padded
+    sizes below 2*ghost and blocks of 2^31 or more cells (the tile kernels
+    index cells in 32 bits) return FAMRDP_E_INVALID with a message."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built (run __graft_entry__.build())")
+    L = _lib.load()
+    gas = _lib.Gas(1.4, 1.0 / 1.4, 1e-3, 1.0, 0.38, 0.72)
+    for dims, msg in (((3, 10, 10), b"2*ghost"), ((2048, 1100, 1000), b"2^31")):
+        plan = _lib.vp()
+        d = (_lib.i32 * 3)(*dims)
+        rc = L.famrdp_plan_create(ctypes.byref(plan), 1, d, 0, None, 0, None, ctypes.byref(gas), 0, 2, 0)
+        assert rc == _lib.E_INVALID and plan.value is None
+        assert msg in L.famrdp_last_error()
+
+
 def test_float64_guard():
     import numpy as np
     with pytest.raises(TypeError):

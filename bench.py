@@ -218,6 +218,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the C4/C5 forward+adjoint measurement (its own JSON line; for profiling the reverse)")
     ap.add_argument("--tile", type=int, default=-1, help="stage-kernel variant (default: autotune)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 2 GiB float64 buffer 5 times (counter calibration)")
@@ -240,6 +242,9 @@ def main():
         return
     from famrdp_jax import _lib
     _lib.load()
+    if args.adjoint_only:
+        print(json.dumps(dict(adjoint=bench_adjoint())), flush=True)
+        return
     c2 = bench_c2(args.n, args.steps, args.warmup, tile=args.tile)
     # HBM bytes per stage launch of the chosen variant, measured with rocprofv3
     # counters for this library version and size (profiles/stage_kernel_traffic.json)

@@ -243,7 +243,7 @@ FD void roe_flux_prim_vjp(const double pl[5], const double pr[5], const double n
   // unh = uh.n
   b_uh += b_unh * n[0]; b_vh += b_unh * n[1]; b_wh += b_unh * n[2];
   // ch = sqrt(c2), rc2 = 1/c2, c2 = max(c2raw, 1e-30)
-  double b_c2 = b_ch * 0.5 / ch - b_rc2 * rc2 * rc2;
+  double b_c2 = b_ch * 0.5 * (ch * rc2) - b_rc2 * rc2 * rc2;   // 1/ch = ch/c2
   const double b_c2raw = b_c2 * dmax_floor(c2raw, 1e-30);
   b_Hh += gm1 * b_c2raw; b_q2 -= 0.5 * gm1 * b_c2raw;
   // q2 = |uh|^2
@@ -259,7 +259,7 @@ FD void roe_flux_prim_vjp(const double pl[5], const double pr[5], const double n
   const double b_S = -b_rs * rs * rs * dmax_floor(S, 1e-150);
   b_sqL += b_S; b_sqR += b_S;
   // sq = sqrt(rho)
-  b_rhoL += b_sqL * 0.5 / sqL; b_rhoR += b_sqR * 0.5 / sqR;
+  b_rhoL += b_sqL * 0.5 * (sqL * irL); b_rhoR += b_sqR * 0.5 * (sqR * irR);   // 1/sqrt(rho) = sqrt(rho)/rho
   // H = (E + p)/rho
   b_EL += b_HL * irL; b_pL += b_HL * irL; b_rhoL -= b_HL * HL * irL;
   b_ER += b_HR * irR; b_pR += b_HR * irR; b_rhoR -= b_HR * HR * irR;

@@ -208,6 +208,27 @@ def test_tile_variants_single_wrap_blocks_vs_oracle(n):
                 np.testing.assert_array_equal(out, fused)
 
 
+@pytest.mark.parametrize("n", [1, 2])
+def test_tile_variants_degenerate_blocks_vs_oracle(n):
+    """Blocks with one or two interior cells (5^3 and 6^3 padded): every tile
+    of every variant overhangs the block several times over along each axis
+    (wrapped by the modulo form), no cell or only one gets the inviscid
+    update; one step per variant against the oracle."""
+    print(f"case: C1 {n}^3 interior, every tile variant vs oracle, 1 step", flush=True)
+    state, cfg = cases.case_c1(n=n, seed=3)
+    m = state.metrics[0]
+    dt = OM.cfl_dt(state.blocks[0].q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+    ref = _ostep([state.blocks[0].q], state, cfg, dt)[0]
+    for v in range(_lib.NUM_TILE_VARIANTS):
+        st = Stepper(to_dev(state), cfg)
+        st.plan.set_tile(v)
+        st.advance(dt, batched=(v % 2 == 1))
+        torch.cuda.synchronize()
+        err = rel_linf(st.qs[0].cpu().numpy(), ref)
+        print(f"  variant {v}: rel Linf {err:.3e}", flush=True)
+        assert err <= STEP_TOL
+
+
 def test_c5_ensemble_per_block_mu0_and_dt():
     """Independent periodic cases with their own Mach, Reynolds number (mu0)
     and CFL time step, advanced together by one batched plan

@@ -83,7 +83,7 @@ def bench_c2(n, steps, warmup, timing_steps=3, tile=-1):
         log(f"C2 {n}^3: dt={dt:.4e}; stage-kernel variant {v} (tile {dims}) as requested")
     else:
         v, dims, ms = st.autotune(dt)
-        log(f"C2 {n}^3: dt={dt:.4e}; autotuned tile {dims} (variant {v}); stage-1 ms per variant "
+        log(f"C2 {n}^3: dt={dt:.4e}; autotuned tile {dims} (variant {v}); stage 1+2 ms per variant "
             + " ".join(f"{x:.3f}" for x in ms))
     st.advance(dt, warmup, graph=True)
     _sync()

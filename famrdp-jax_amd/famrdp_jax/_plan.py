@@ -120,9 +120,9 @@ class Plan:
 
     def autotune(self, qs, kxyzs, vols, dt, *, batched=False, reps=3):
         """Time every stage-kernel variant (tile shapes of the fused kernel and
-        of the split pair) on these arrays (stage 1 only, the state is not
-        modified) and keep the fastest.  Returns (chosen variant, tile dims,
-        per-variant ms)."""
+        of the split pair) on these arrays (stages 1 and 2, which write only
+        the plan's workspace: the state is not modified) and keep the fastest.
+        Returns (chosen variant, tile dims, per-variant ms of stages 1 + 2)."""
         n = L.NUM_TILE_VARIANTS
         ms = (L.f64 * n)()
         L.check(self._lib.famrdp_plan_autotune(self._h, self._ptrs(qs), self._ptrs(kxyzs), self._ptrs(vols), float(dt),

@@ -138,10 +138,11 @@ int famrdp_step_graph(famrdp_plan* plan, double* const* q, const double* const* 
  * and 10-13 are tile shapes of the fused stage kernel (4 and 10-13 stage the
  * primitives twice to need less LDS), 5-9 tile shapes of the split pair
  * (viscous tensor kernel + stage kernel).  famrdp_plan_autotune times `reps`
- * stage-1 launches of every variant for these arrays (stage 1 writes only the
- * plan's workspace, so the state is not modified), writes the per-variant ms
- * to ms_out (may be NULL; FAMRDP_NUM_TILE_VARIANTS entries) and keeps the
- * fastest. */
+ * stage-1 + stage-2 launch pairs of every variant for these arrays (both write
+ * only the plan's workspace, so the state is not modified; stage 2 also reads
+ * q0 like stage 3), writes the per-variant ms of one pair to ms_out (may be
+ * NULL; FAMRDP_NUM_TILE_VARIANTS entries) and keeps the fastest; a variant
+ * within 0.5 % of an earlier one counts as a tie and the earlier one stays. */
 #define FAMRDP_NUM_TILE_VARIANTS 14
 int famrdp_plan_autotune(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                          double dt, int32_t flags, int32_t reps, double* ms_out, void* stream);

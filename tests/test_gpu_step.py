@@ -208,6 +208,40 @@ def test_tile_variants_single_wrap_blocks_vs_oracle(n):
                 np.testing.assert_array_equal(out, fused)
 
 
+def test_tile_variants_vanleer_vs_oracle():
+    """The van Leer limiter (a template instance of every stage kernel of its
+    own): every variant, plain and batched, one step against the oracle on
+    the C2 grid at 14^3 (18^3 padded, single wrap) and on the small uneven
+    multi-block case (modulo wrap); fused variants bitwise among themselves."""
+    from famrdp_jax.core.types import Config, SchemeChoice
+    print("case: van Leer, every tile variant vs oracle (C2 grid 14^3, 2x2x2 blocks), 1 step", flush=True)
+    for state, cfg in (_c2_small(14)[:2], _small_multiblock()):
+        cfg = Config(gas=cfg.gas, scheme=SchemeChoice("muscl2", "roe", 3, "vanleer"), bc_params=cfg.bc_params,
+                     topology=cfg.topology, ghost=2)
+        dt = 0.5 * min(OM.cfl_dt(b.q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)
+                       for b, m in zip(state.blocks, state.metrics))
+        mets = [(m.kxyz, m.vol) for m in state.metrics]
+        ref = OB.step([b.q for b in state.blocks], mets, list(cfg.topology), cfg.bc_params, cases.gas_dict(cfg.gas),
+                      dt, limiter="vanleer")
+        fused = None
+        for v in range(_lib.NUM_TILE_VARIANTS):
+            for batched in (False, True):
+                st = Stepper(to_dev(state), cfg)
+                st.plan.set_tile(v)
+                st.advance(dt, batched=batched)
+                torch.cuda.synchronize()
+                out = [q.cpu().numpy() for q in st.qs]
+                err = max(rel_linf(a, b) for a, b in zip(out, ref))
+                assert err <= STEP_TOL, (v, batched, err)
+                if v in _lib.FUSED_VARIANTS:
+                    if fused is None:
+                        fused = out
+                    else:
+                        for a, b in zip(out, fused):
+                            np.testing.assert_array_equal(a, b)
+        print(f"  {len(state.blocks)} block(s): every variant within {STEP_TOL:g}", flush=True)
+
+
 @pytest.mark.parametrize("n", [1, 2])
 def test_tile_variants_degenerate_blocks_vs_oracle(n):
     """Blocks with one or two interior cells (5^3 and 6^3 padded): every tile

@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the C4/C5 forward+adjoint measurement (its own JSON line; for profiling the reverse)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the C3 batched vs per-block measurement (its own JSON line; for profiling it)")
     ap.add_argument("--tile", type=int, default=-1, help="stage-kernel variant (default: autotune)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 2 GiB float64 buffer 5 times (counter calibration)")
@@ -244,6 +246,9 @@ def main():
     _lib.load()
     if args.adjoint_only:
         print(json.dumps(dict(adjoint=bench_adjoint())), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps(dict(batched=bench_c3())), flush=True)
         return
     c2 = bench_c2(args.n, args.steps, args.warmup, tile=args.tile)
     # HBM bytes per stage launch of the chosen variant, measured with rocprofv3

@@ -29,7 +29,7 @@ __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double*
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
   const int k = (int)(a % nk), j = (int)((a / nk) % nj), i = (int)(a / ((int64_t)nk * nj));
-#pragma unroll 1
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     double* out = b.fs + (int64_t)(d * 10) * N + a;
     const int ad = d == 0 ? i : (d == 1 ? j : k);

@@ -107,29 +107,42 @@ def bench_c2(n, steps, warmup, timing_steps=3, tile=-1):
                 achieved_gbs=achieved, finite=finite)
 
 
-def cpu_baseline(n=80, steps=10):
-    """Oracle step (numpy, one core) on the same curvilinear TGV workload at
-    n^3 interior: 1 warm-up step, then ``steps`` timed steps."""
+def cpu_baseline(n=112, target_s=10.0):
+    """The reference's step on the host CPU: the oracle's C + OpenMP
+    restatement on every host thread (oracle/c_step.py, pinned against the
+    numpy oracle) on the same curvilinear TGV workload at n^3 interior; one
+    warm-up step, then as many steps as fill about ``target_s`` seconds.
+    Without the built C oracle, the numpy oracle on one core."""
     import numpy as np
     from famrdp_jax import cases
     from oracle import blocks as OB
+    from oracle import c_step
     from oracle import metric as OM
     q, xyz_ext, cfg = cases.case_c2(n)
     jac, k, vol = OM.compute_metrics(xyz_ext)
     m = cases.crop_metrics(jac, k, vol)
     gas = cases.gas_dict(cfg.gas)
     dt = OM.cfl_dt(q, m.kxyz, m.vol, gas, 0.5)
-    mets = [(m.kxyz, m.vol)]
-    topo = list(cfg.topology)
-    qs = OB.step([q], mets, topo, cfg.bc_params, gas, dt)
+    if c_step.available():
+        # the job's CPU share: OMP_NUM_THREADS where set, else at most 16 threads
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(16, os.cpu_count() or 1)
+        one = lambda x: c_step.step(x, m.kxyz, m.vol, cfg.topology[0], gas, dt, threads=threads)
+        what, cores = f"oracle C + OpenMP step ({threads} threads)", threads
+    else:
+        mets, topo = [(m.kxyz, m.vol)], list(cfg.topology)
+        one = lambda x: OB.step([x], mets, topo, cfg.bc_params, gas, dt)[0]
+        what, cores = "oracle numpy step (C oracle not built)", 1
+    t0 = time.perf_counter()
+    q = one(q)                                   # warm-up, also sizes the sample
+    steps = max(3, min(50, int(target_s / max(time.perf_counter() - t0, 1e-3))))
     t0 = time.perf_counter()
     for _ in range(steps):
-        qs = OB.step(qs, mets, topo, cfg.bc_params, gas, dt)
+        q = one(q)
     wall = time.perf_counter() - t0
-    assert np.isfinite(qs[0]).all()
-    return dict(value=n ** 3 * steps / wall, unit="cell-updates/s", cores=1, kind="port",
-                sample=f"oracle numpy step on the same curvilinear TGV workload at {n}^3 interior "
-                       f"({n + 4}^3 padded), {steps} steps after 1 warm-up, {wall:.1f} s",
+    assert np.isfinite(q).all()
+    return dict(value=n ** 3 * steps / wall, unit="cell-updates/s", cores=cores, kind="port",
+                sample=f"{what} on the same curvilinear TGV workload at {n}^3 interior ({n + 4}^3 padded), "
+                       f"{steps} steps after 1 warm-up, {wall:.1f} s",
                 host_cpus=os.cpu_count(), omp_threads=os.environ.get("OMP_NUM_THREADS"))
 
 

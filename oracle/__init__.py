@@ -14,6 +14,11 @@ Contents (each function cites the reference file:line it restates):
 * ``blocks``: halo exchange, BCs, the full ``step``.
 * ``metric``: 4th-order metrics, Fortran probe readers, the build-defined CFL
   time step.
+* ``c_step`` (with ``c/step_omp.c``): the full ``step`` of one periodic block
+  restated in C + OpenMP in the numpy oracle's evaluation order, for the CPU
+  baseline on every host thread; pinned against ``blocks.step``
+  (``tests/test_oracle_c.py``).  Not imported here (it needs its built
+  library).
 
 Pinning (what checks each function independently of its own formula) is
 listed in ``tests/test_oracle_*.py``; every function above is pinned by at

@@ -142,7 +142,7 @@ int famrdp_step_graph(famrdp_plan* plan, double* const* q, const double* const* 
  * only the plan's workspace, so the state is not modified; stage 2 also reads
  * q0 like stage 3), writes the per-variant ms of one pair to ms_out (may be
  * NULL; FAMRDP_NUM_TILE_VARIANTS entries) and keeps the fastest; a variant
- * within 0.5 % of an earlier one counts as a tie and the earlier one stays. */
+ * within 1.5 % of an earlier one counts as a tie and the earlier one stays. */
 #define FAMRDP_NUM_TILE_VARIANTS 14
 int famrdp_plan_autotune(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                          double dt, int32_t flags, int32_t reps, double* ms_out, void* stream);

@@ -8,6 +8,7 @@
 //   * halo gather/scatter (pre-exchange semantics) and ghost-cell BC faces;
 //   * grid metrics and the CFL / positivity reduction.
 #include "famrdp_kernels.h"
+#include "famrdp_tiles.h"
 
 namespace famrdp {
 
@@ -570,126 +571,6 @@ __global__ void __launch_bounds__(256) k_cfl_final(const double* __restrict__ pa
 // ===========================================================================
 namespace famrdp {
 
-// Periodic (roll) wrap of tile coordinates into a block and the cell index.
-// Tiles start inside the block (origin <= n - 1) and the cells a tile kernel
-// touches lie at most 2 cells before and T + 1 past its origin along an axis
-// of tile extent T, so in a block whose padded sizes are all >= WRAP_ONCE_MIN
-// > T a coordinate lies in [-n, 2n) and one conditional add or subtract wraps
-// it; written as unsigned minima it takes two full-rate instructions and no
-// branch.  With nj*nk < 2^24 the index then comes from 24-bit multiplies.
-// Other blocks (BlockDesc::wrap_mod) take a modulo; the choice is uniform per
-// block.  Plans hold blocks of fewer than 2^31 cells, so indices are 32-bit.
-constexpr int WRAP_ONCE_MIN = 18;
-
-struct Tile3 {
-  int32_t block, i0, j0, k0;
-};
-
-template <int TI, int TJ, int TK>
-struct FusedGeom {
-  static_assert(TI < WRAP_ONCE_MIN && TJ < WRAP_ONCE_MIN && TK < WRAP_ONCE_MIN, "tile longer than the single-wrap bound");
-  static constexpr int NT = TI * TJ * TK;
-  static constexpr int B0J = TJ + 2, B0K = TK + 2;
-  static constexpr int NR0 = (TI + 2) * B0J * B0K;      // box with a one-cell halo
-  static constexpr int NR1 = 2 * TJ * TK;               // second i layer  [xe][y][z]
-  static constexpr int NR2 = 2 * TI * TK;               // second j layer  [ye][x][z]
-  static constexpr int NR3 = 2 * TI * TJ;               // second k layer  [ze][x][y]
-  static constexpr int NPV = NR0 + NR1 + NR2 + NR3;
-  static constexpr int NG = NT + 2 * (TJ * TK + TI * TK + TI * TJ);   // tile + cross halo 1
-  static constexpr int F0 = (TI + 1) * TJ * TK, F1 = TI * (TJ + 1) * TK, F2 = TI * TJ * (TK + 1);
-  static constexpr int NF = F0 > F1 ? (F0 > F2 ? F0 : F2) : (F1 > F2 ? F1 : F2);
-  // primitives (5) and temperature per staged cell, then G or the faces
-  static constexpr int NLDS = 6 * NPV + (9 * NG > 5 * NF ? 9 * NG : 5 * NF);
-  // re-staging form: G overwrites the primitives (which are staged again for
-  // the faces), so LDS holds the larger of the three phases only
-  static constexpr int NLDS_RS_A = 6 * NPV > 9 * NG ? 6 * NPV : 9 * NG;
-  static constexpr int NLDS_RS = NLDS_RS_A > 5 * NPV + 5 * NF ? NLDS_RS_A : 5 * NPV + 5 * NF;
-  static constexpr int NPVS = NPV;   // primitive slots (stride of a variable)
-
-  // slot of the cell (lx, ly, lz) whose coordinate along D lies in
-  // [-2, T_D + 1] and whose other two coordinates lie in the tile
-  template <int D>
-  __device__ static __forceinline__ int axis_slot(int lx, int ly, int lz) {
-    if constexpr (D == 0) {
-      const bool out = (unsigned)(lx + 1) > (unsigned)(TI + 1);
-      return out ? NR0 + ((lx < 0 ? 0 : 1) * TJ + ly) * TK + lz : ((lx + 1) * B0J + (ly + 1)) * B0K + (lz + 1);
-    } else if constexpr (D == 1) {
-      const bool out = (unsigned)(ly + 1) > (unsigned)(TJ + 1);
-      return out ? NR0 + NR1 + ((ly < 0 ? 0 : 1) * TI + lx) * TK + lz : ((lx + 1) * B0J + (ly + 1)) * B0K + (lz + 1);
-    } else {
-      const bool out = (unsigned)(lz + 1) > (unsigned)(TK + 1);
-      return out ? NR0 + NR1 + NR2 + ((lz < 0 ? 0 : 1) * TI + lx) * TJ + ly
-                 : ((lx + 1) * B0J + (ly + 1)) * B0K + (lz + 1);
-    }
-  }
-
-  // (written with selects only: no divergent branches in the face loops)
-  __device__ static __forceinline__ int pv_slot(int x, int y, int z) {
-    const bool ox = (unsigned)(x + 1) > (unsigned)(TI + 1), oy = (unsigned)(y + 1) > (unsigned)(TJ + 1);
-    const bool oz = (unsigned)(z + 1) > (unsigned)(TK + 1);
-    const int s_box = ((x + 1) * B0J + (y + 1)) * B0K + (z + 1);
-    const int s_x = NR0 + ((x < 0 ? 0 : 1) * TJ + y) * TK + z;
-    const int s_y = NR0 + NR1 + ((y < 0 ? 0 : 1) * TI + x) * TK + z;
-    const int s_z = NR0 + NR1 + NR2 + ((z < 0 ? 0 : 1) * TI + x) * TJ + y;
-    return ox ? s_x : (oy ? s_y : (oz ? s_z : s_box));
-  }
-  // local coordinates of LDS primitive slot h (inverse of pv_slot)
-  __device__ static __forceinline__ int3 pv_cell(int h) {
-    if (h < NR0) return make_int3(h / (B0J * B0K) - 1, (h / B0K) % B0J - 1, h % B0K - 1);
-    h -= NR0;
-    if (h < NR1) return make_int3(h / (TJ * TK) ? TI + 1 : -2, (h / TK) % TJ, h % TK);
-    h -= NR1;
-    if (h < NR2) return make_int3((h / TK) % TI, h / (TI * TK) ? TJ + 1 : -2, h % TK);
-    h -= NR2;
-    return make_int3((h / TJ) % TI, h % TJ, h / (TI * TJ) ? TK + 1 : -2);
-  }
-  __device__ static __forceinline__ int g_slot(int x, int y, int z) {
-    const bool ox = (unsigned)x >= (unsigned)TI, oy = (unsigned)y >= (unsigned)TJ, oz = (unsigned)z >= (unsigned)TK;
-    const int s_in = (x * TJ + y) * TK + z;
-    const int s_x = NT + ((x < 0 ? 0 : 1) * TJ + y) * TK + z;
-    const int s_y = NT + 2 * TJ * TK + ((y < 0 ? 0 : 1) * TI + x) * TK + z;
-    const int s_z = NT + 2 * TJ * TK + 2 * TI * TK + ((z < 0 ? 0 : 1) * TI + x) * TJ + y;
-    return ox ? s_x : (oy ? s_y : (oz ? s_z : s_in));
-  }
-  // local coordinates of G slot h (inverse of g_slot)
-  __device__ static __forceinline__ int3 g_cell(int h) {
-    if (h < NT) return make_int3(h / (TJ * TK), (h / TK) % TJ, h % TK);
-    h -= NT;
-    if (h < 2 * TJ * TK) return make_int3(h / (TJ * TK) ? TI : -1, (h / TK) % TJ, h % TK);
-    h -= 2 * TJ * TK;
-    if (h < 2 * TI * TK) return make_int3((h / TK) % TI, h / (TI * TK) ? TJ : -1, h % TK);
-    h -= 2 * TI * TK;
-    return make_int3((h / TJ) % TI, h % TJ, h / (TI * TJ) ? TK : -1);
-  }
-};
-
-
-__host__ __device__ inline int tile_wrap_mod(int ni, int nj, int nk) {
-  return (ni < WRAP_ONCE_MIN || nj < WRAP_ONCE_MIN || nk < WRAP_ONCE_MIN || (int64_t)nj * nk >= (1 << 24)) ? 1 : 0;
-}
-
-__device__ __forceinline__ int wrap_once(int i, int n) {   // i in [-n, 2n)
-  const unsigned a = min((unsigned)i, (unsigned)(i + n));
-  return (int)min(a, a - (unsigned)n);
-}
-
-__device__ __forceinline__ int wrap_mod(int i, int n) {
-  const int r = i % n;
-  return r < 0 ? r + n : r;
-}
-
-// index of cell (i, j, k) of the block, each coordinate wrapped
-template <bool MOD>
-__device__ __forceinline__ int tile_cell(int i, int j, int k, int ni, int nj, int nk) {
-  if constexpr (MOD) {
-    return (wrap_mod(i, ni) * nj + wrap_mod(j, nj)) * nk + wrap_mod(k, nk);
-  } else {
-    return (int)(__umul24((unsigned)wrap_once(i, ni), (unsigned)(nj * nk)) +
-                 __umul24((unsigned)wrap_once(j, nj), (unsigned)nk)) +
-           wrap_once(k, nk);
-  }
-}
-
 // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so map
 // consecutive ids of one XCD onto consecutive tiles (neighbouring tiles share
 // halo lines in that XCD's L2).
@@ -786,22 +667,7 @@ __device__ __forceinline__ void face_axis(const BlockDesc& b, int i0, int j0, in
   __syncthreads();
 }
 
-// Cell indices a thread of the fused kernel loads from: its staging slots
-// t, t + NT, ... (indices past the last slot clamped to it, the values
-// unused), the G cells beyond the tile it computes, and its own tile cell
-// (wrapped into the block when outside it).  Computed once per tile; the
-// wrap form is chosen per block by a uniform branch.
-template <int TI, int TJ, int TK>
-struct FusedCells {
-  using T = FusedGeom<TI, TJ, TK>;
-  static constexpr int NIT = (T::NPV + T::NT - 1) / T::NT;
-  static constexpr int NGIT = (T::NG + T::NT - 1) / T::NT;
-  int stage[NIT];
-  int gh[NGIT > 1 ? NGIT - 1 : 1];
-  int c;
-};
-
-template <int TI, int TJ, int TK, bool MOD, bool ALL>
+template <int TI, int TJ, int TK, int W, bool ALL>
 __device__ __forceinline__ void fused_cells_fill(const BlockDesc& b, int i0, int j0, int k0, int t,
                                                  FusedCells<TI, TJ, TK>& fc) {
   using C = FusedCells<TI, TJ, TK>;
@@ -810,26 +676,28 @@ __device__ __forceinline__ void fused_cells_fill(const BlockDesc& b, int i0, int
 #pragma unroll
   for (int it = 0; it < C::NIT; ++it) {
     const int h = t + it * T::NT;
-    const int3 lc = T::pv_cell(h < T::NPV ? h : T::NPV - 1);
-    fc.stage[it] = tile_cell<MOD>(i0 + lc.x, j0 + lc.y, k0 + lc.z, ni, nj, nk);
+    const Loc3 l = T::pv_cell(h < T::NPV ? h : T::NPV - 1);
+    fc.stage[it] = tile_cell_rel<W>(l.x + 2, l.y + 2, l.z + 2, i0, j0, k0, ni, nj, nk);
   }
   if constexpr (!ALL) return;
 #pragma unroll
   for (int it = 1; it < C::NGIT; ++it) {
     const int h = t + it * T::NT;
-    const int3 lc = T::g_cell(h < T::NG ? h : T::NG - 1);
-    fc.gh[it - 1] = tile_cell<MOD>(i0 + lc.x, j0 + lc.y, k0 + lc.z, ni, nj, nk);
+    const Loc3 l = T::g_cell(h < T::NG ? h : T::NG - 1);
+    fc.gh[it - 1] = tile_cell_rel<W>(l.x + 2, l.y + 2, l.z + 2, i0, j0, k0, ni, nj, nk);
   }
   const int x = t / (TJ * TK), y = (t / TK) % TJ, z = t % TK;
-  fc.c = tile_cell<MOD>(i0 + x, j0 + y, k0 + z, ni, nj, nk);
+  fc.c = tile_cell<W>(i0 + x, j0 + y, k0 + z, ni, nj, nk);
 }
 
 // ALL = false: the staging slots only, for thread id t
 template <int TI, int TJ, int TK, bool ALL = true>
 __device__ __forceinline__ void fused_cells(const BlockDesc& b, int i0, int j0, int k0, int t,
                                             FusedCells<TI, TJ, TK>& fc) {
-  if (b.wrap_mod) fused_cells_fill<TI, TJ, TK, true, ALL>(b, i0, j0, k0, t, fc);
-  else fused_cells_fill<TI, TJ, TK, false, ALL>(b, i0, j0, k0, t, fc);
+  const int w = fused_wrap_mode<TI, TJ, TK>(b.wrap_mod, i0, j0, k0, b.ni, b.nj, b.nk);
+  if (w == WRAP_MOD) fused_cells_fill<TI, TJ, TK, WRAP_MOD, ALL>(b, i0, j0, k0, t, fc);
+  else if (w == WRAP_NONE) fused_cells_fill<TI, TJ, TK, WRAP_NONE, ALL>(b, i0, j0, k0, t, fc);
+  else fused_cells_fill<TI, TJ, TK, WRAP_ONCE, ALL>(b, i0, j0, k0, t, fc);
 }
 
 // v, hidden from the optimiser: values computed from it are computed again
@@ -917,7 +785,7 @@ __device__ __forceinline__ void stage_tile_fused_regs(const BlockDesc& b, int i0
   for (int it = 0; it < NGIT; ++it) {
     const int h = t + it * T::NT;
     if (h >= T::NG) break;
-    const int3 lc = T::g_cell(h);
+    const Loc3 lc = T::g_cell(h);
     const int lx = lc.x, ly = lc.y, lz = lc.z;
     double kc[9];
 #pragma unroll
@@ -1149,7 +1017,7 @@ __device__ __forceinline__ void visc_tile(const BlockDesc& b, int i0, int j0, in
 #pragma unroll
       for (int it = 0; it < NIT; ++it) {
         const int h = t + it * T::NT;
-        const int3 lc = T::g_cell(h < T::NG ? h : T::NG - 1);
+        const Loc3 lc = T::g_cell(h < T::NG ? h : T::NG - 1);
         cc[it] = tile_cell<decltype(mod)::value>(i0 + lc.x, j0 + lc.y, k0 + lc.z, ni, nj, nk);
       }
     };

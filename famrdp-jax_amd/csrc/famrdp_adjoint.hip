@@ -1,13 +1,14 @@
 // Adjoint (reverse-mode) kernels of the TVD-RK3 step.
 //
-// One RK stage  q_out = a q0 + b (q_s + dt L(q_s))  is reversed by three
+// One RK stage  q_out = a q0 + b (q_s + dt L(q_s))  is reversed by two
 // gather-form kernels (no atomics, deterministic):
-//   k_vjp_faces   every inviscid face: cotangent of F*|n| from the two cells
-//                 that difference it, Roe + Harten reverse sweep, stored as
-//                 (d/d pL, d/d pR) at the face's left cell;
-//   k_vjp_visc    every cell x: cotangent of the viscous flux tensor G(x)
+//   k_vjp_faces_visc
+//                 every cell x: cotangent of the viscous flux tensor G(x)
 //                 from the divergence at its six neighbours, reverse sweep of
-//                 G into d/d(u, v, w, T) and d/d(central differences);
+//                 G into d/d(u, v, w, T) and d/d(central differences); then
+//                 every inviscid face whose left cell is x: cotangent of
+//                 F*|n| from the two cells that difference it, Roe + Harten
+//                 reverse sweep, stored as (d/d pL, d/d pR) at x;
 //   k_vjp_cell    every cell y: gathers the face states, the MUSCL slopes of
 //                 y-1, y, y+1 (limiter reverse), the central differences of
 //                 its neighbours and T, maps the primitive cotangent to
@@ -222,9 +223,14 @@ __device__ __forceinline__ double stage_cmu(int stage, double dt) {
   return stage == 3 ? (2.0 / 3.0) * dt : (stage == 2 ? 0.25 * dt : dt);
 }
 
+// The viscous part first: the neighbours' state and metrics it loads are
+// then in cache for the face part.  The scheduling barrier keeps the two
+// parts' instructions apart, and the face part's cell index is hidden from
+// value numbering so no value loaded by the viscous part is held in registers
+// through the face part (two waves per SIMD, as the face part alone needs).
 template <int LIM>
-__global__ void __launch_bounds__(256) k_vjp_faces(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
-                                                   int stage, double g, double dt) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_vjp_faces_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int stage, Gas gas, double dt) {
   const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
   const BlockDesc& b = blocks[tl.block];
   const int64_t c = (int64_t)tl.start + threadIdx.x;
@@ -232,22 +238,15 @@ __global__ void __launch_bounds__(256) k_vjp_faces(const BlockDesc* __restrict__
   const double *qs, *lin;
   double* lout;
   vjp_bufs(b, stage, qs, lin, lout);
-  cell_vjp_faces<LIM>(b, qs, lin, stage_cmu(stage, dt * b.dt_scale), c, g);
+  const double cmu = stage_cmu(stage, dt * b.dt_scale);
+  cell_vjp_visc(b, qs, lin, cmu, c, gas);
+  __builtin_amdgcn_sched_barrier(0);
+  int cf = (int)c;   // c < N < 2^31
+  asm volatile("" : "+v"(cf));
+  cell_vjp_faces<LIM>(b, qs, lin, cmu, (int64_t)cf, gas.gamma);
 }
 
-// at most 128 VGPRs: these memory-bound gathers want four waves per SIMD
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vjp_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
-                                                  int stage, Gas gas, double dt) {
-  const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
-  const BlockDesc& b = blocks[tl.block];
-  const int64_t c = (int64_t)tl.start + threadIdx.x;
-  if (c >= b.N) return;
-  const double *qs, *lin;
-  double* lout;
-  vjp_bufs(b, stage, qs, lin, lout);
-  cell_vjp_visc(b, qs, lin, stage_cmu(stage, dt * b.dt_scale), c, gas);
-}
-
+// at most 128 VGPRs: this memory-bound gather wants four waves per SIMD
 template <int LIM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vjp_cell(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
                                                   int stage, Gas gas) {

@@ -302,7 +302,7 @@ extern "C" {
 
 const char* famrdp_last_error(void) { return g_last_error.c_str(); }
 
-const char* famrdp_version(void) { return "famrdp-mi355x 0.6 (gfx950, fp64, fused and split stage kernels, autotuned)"; }
+const char* famrdp_version(void) { return "famrdp-mi355x 0.7 (gfx950, fp64, fused and split stage kernels, autotuned)"; }
 
 int famrdp_cons_to_prim(const double* q, double* pv, int64_t n, double gamma, void* stream) {
   if (n < 0 || (n > 0 && (!q || !pv))) return fail(FAMRDP_E_INVALID, "cons_to_prim: bad arguments");
@@ -776,11 +776,9 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
   if (rc) return rc;
   for (int stage = 3; stage >= 1; --stage) {
     if (p->limiter == LIM_VANLEER)
-      k_vjp_faces<LIM_VANLEER><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas.gamma, dt);
+      k_vjp_faces_visc<LIM_VANLEER><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, dt);
     else
-      k_vjp_faces<LIM_MINMOD><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas.gamma, dt);
-    LAUNCH_CHECK();
-    k_vjp_visc<<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, dt);
+      k_vjp_faces_visc<LIM_MINMOD><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, dt);
     LAUNCH_CHECK();
     if (p->limiter == LIM_VANLEER)
       k_vjp_cell<LIM_VANLEER><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas);

@@ -3,10 +3,11 @@
 The reference differentiates its eager step with jax.grad
 (tests/solver/test_grad.py:10-38), storing every intermediate.  Here the
 reverse pass is famrdp_step_vjp (HIP), driven with checkpointing: the forward
-keeps the state every ``checkpoint_every`` steps; the reverse pass recomputes
-one segment at a time (keeping its step-start states), and each step's
-reverse recomputes its halo/BC state and RK stages 1-2 before reversing
-stages 3, 2, 1, the BCs and the halo copy.
+keeps the state every ``checkpoint_every`` steps and, for the last segment
+(where the reverse pass starts), every step's start and stage states; the
+reverse pass recomputes the other segments one at a time (keeping their
+step-start and stage states), and each step's reverse rebuilds its halo/BC
+state before reversing stages 3, 2, 1, the BCs and the halo copy.
 
 Besides d J / d q0, the farfield states q_inf receive their cotangents
 (summed over steps) per (block, face); ``mach_derivative`` turns them into
@@ -60,15 +61,30 @@ class Adjoint:
             self._step(qs, dt)
         return qs
 
-    def forward_checkpointed(self, qs, dt, nsteps, checkpoint_every):
+    def forward_checkpointed(self, qs, dt, nsteps, checkpoint_every, keep_last_segment=False):
         """Advance copies of ``qs``; returns (final states, {step: states})
-        with the states kept every ``checkpoint_every`` steps (step 0 included)."""
+        with the states kept every ``checkpoint_every`` steps (step 0 included).
+        keep_last_segment also keeps the last segment's step-start and stage
+        states, returned third as (first step of the segment, starts, stages),
+        so that the reverse pass does not recompute that segment."""
         cur = [q.detach().clone() for q in qs]
         checkpoints = {0: [q.clone() for q in cur]}
+        last_start = (nsteps - 1) // checkpoint_every * checkpoint_every if nsteps > 0 else 0
+        starts, stages = [], []
         for n in range(nsteps):
+            keep = keep_last_segment and n >= last_start
+            if keep:
+                starts.append([q.clone() for q in cur])
             self._step(cur, dt)
+            if keep:
+                q1s = [torch_empty_like(c) for c in cur]
+                q2s = [torch_empty_like(c) for c in cur]
+                self.plan.stage_states(q1s, q2s)
+                stages.append((q1s, q2s))
             if (n + 1) % checkpoint_every == 0 and n + 1 < nsteps:
                 checkpoints[n + 1] = [q.clone() for q in cur]
+        if keep_last_segment:
+            return cur, checkpoints, (last_start, starts, stages)
         return cur, checkpoints
 
     def objective(self, qs, kind, blocks=None):
@@ -86,27 +102,36 @@ class Adjoint:
             J += float(self._J.item())
         return J, grads
 
-    def reverse(self, lam, dt, nsteps, checkpoints, checkpoint_every, dqinf, *, keep_stages=True):
+    def reverse(self, lam, dt, nsteps, checkpoints, checkpoint_every, dqinf, *, keep_stages=True, last=None):
         """lam (cotangent of the final state) -> cotangent of q0, in place.
         Each segment is recomputed from its checkpoint keeping every step's
         start state and (keep_stages) its stage-1/2 states, so the reverse of
-        a step does not recompute its stages."""
+        a step does not recompute its stages; ``last`` (from
+        forward_checkpointed(..., keep_last_segment=True)) supplies the last
+        segment's states instead of a recomputation."""
         k = checkpoint_every
         for seg_start in reversed(range(0, nsteps, k)):
             seg_end = min(seg_start + k, nsteps)
-            cur = [c.clone() for c in checkpoints[seg_start]]
-            starts, stages = [], []
-            for _ in range(seg_start, seg_end):
-                starts.append([c.clone() for c in cur])
-                self._step(cur, dt)
-                if keep_stages:
-                    q1s = [torch_empty_like(c) for c in cur]
-                    q2s = [torch_empty_like(c) for c in cur]
-                    self.plan.stage_states(q1s, q2s)
-                    stages.append((q1s, q2s))
+            if last is not None and keep_stages and last[0] == seg_start:
+                _, starts, stages = last
+                last = None
+            else:
+                cur = [c.clone() for c in checkpoints[seg_start]]
+                starts, stages = [], []
+                for _ in range(seg_start, seg_end):
+                    starts.append([c.clone() for c in cur])
+                    self._step(cur, dt)
+                    if keep_stages:
+                        q1s = [torch_empty_like(c) for c in cur]
+                        q2s = [torch_empty_like(c) for c in cur]
+                        self.plan.stage_states(q1s, q2s)
+                        stages.append((q1s, q2s))
             for n in reversed(range(seg_end - seg_start)):
                 self.plan.step_vjp(starts[n], self.kxyz, self.vol, lam, dt, dqinf, batched=self.batched,
                                    direct=self.direct, stages=stages[n] if keep_stages else None)
+                starts[n] = None
+                if keep_stages:
+                    stages[n] = None    # release as the reverse pass goes
         return lam
 
     def gradient(self, dt, nsteps, kind=OBJ_KINETIC_ENERGY, *, checkpoint_every=10, blocks=None, timing=None):
@@ -135,10 +160,11 @@ class Adjoint:
                                    direct=self.direct, stages=stages[n])
                 starts[n] = stages[n] = None    # release as the reverse pass goes
         else:
-            cur, checkpoints = self.forward_checkpointed(self.q0, dt, nsteps, checkpoint_every)
+            cur, checkpoints, last = self.forward_checkpointed(self.q0, dt, nsteps, checkpoint_every,
+                                                               keep_last_segment=True)
             J, lam = self.objective(cur, kind, blocks)
             t1 = time.perf_counter()
-            self.reverse(lam, dt, nsteps, checkpoints, checkpoint_every, dqinf)
+            self.reverse(lam, dt, nsteps, checkpoints, checkpoint_every, dqinf, last=last)
         torch.cuda.synchronize()
         if timing is not None:
             timing["forward_with_checkpoints_s"] = t1 - t0

@@ -188,6 +188,11 @@ def test_kept_stage_states_equal_recomputed():
     torch.cuda.synchronize()
     np.testing.assert_array_equal(g0[0].cpu().numpy(), outs[0][0])
     np.testing.assert_array_equal(dq0.reshape(-1).cpu().numpy(), outs[0][1])
+    # checkpointed, the last segment's states kept by the forward pass: same bits
+    J2, g2, dq2 = adj.gradient(dt, 3, OBJ_WALL_PRESSURE, checkpoint_every=2)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(g2[0].cpu().numpy(), outs[0][0])
+    np.testing.assert_array_equal(dq2.reshape(-1).cpu().numpy(), outs[0][1])
 
 
 def test_torch_autograd_function():

@@ -223,11 +223,10 @@ __device__ __forceinline__ double stage_cmu(int stage, double dt) {
   return stage == 3 ? (2.0 / 3.0) * dt : (stage == 2 ? 0.25 * dt : dt);
 }
 
-// The viscous part first: the neighbours' state and metrics it loads are
-// then in cache for the face part.  The scheduling barrier keeps the two
-// parts' instructions apart, and the face part's cell index is hidden from
-// value numbering so no value loaded by the viscous part is held in registers
-// through the face part (two waves per SIMD, as the face part alone needs).
+// The viscous part first: the face part reuses the neighbours' state and
+// metrics it loaded (from registers or cache).  The scheduling barrier keeps
+// the two parts' instructions apart (two waves per SIMD, as the face part
+// alone needs; 4 registers spilled).
 template <int LIM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_vjp_faces_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int stage, Gas gas, double dt) {
@@ -241,9 +240,7 @@ k_vjp_faces_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ 
   const double cmu = stage_cmu(stage, dt * b.dt_scale);
   cell_vjp_visc(b, qs, lin, cmu, c, gas);
   __builtin_amdgcn_sched_barrier(0);
-  int cf = (int)c;   // c < N < 2^31
-  asm volatile("" : "+v"(cf));
-  cell_vjp_faces<LIM>(b, qs, lin, cmu, (int64_t)cf, gas.gamma);
+  cell_vjp_faces<LIM>(b, qs, lin, cmu, c, gas.gamma);
 }
 
 // at most 128 VGPRs: this memory-bound gather wants four waves per SIMD

@@ -96,7 +96,7 @@ FD void roe_flux_prim_vjp(const double pl[5], const double pr[5], const double n
   // ---------------- forward (kept intermediates) ----------------
   const double gm1 = g - 1.0;
   const double rgm1 = rcp_hd(gm1);
-  const double rhoL = fmax(pl[0], 1e-300), rhoR = fmax(pr[0], 1e-300);
+  const double rhoL = max_floor(pl[0], 1e-300), rhoR = max_floor(pr[0], 1e-300);
   const double uL = pl[1], vL = pl[2], wL = pl[3], pL = pl[4];
   const double uR = pr[1], vR = pr[2], wR = pr[3], pR = pr[4];
   const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
@@ -106,14 +106,14 @@ FD void roe_flux_prim_vjp(const double pl[5], const double pr[5], const double n
   const double HL = (EL + pL) * irL, HR = (ER + pR) * irR;
   const double sqL = sqrt_hd(rhoL), sqR = sqrt_hd(rhoR);
   const double S = sqL + sqR;
-  const double Sf = fmax(S, 1e-150);
+  const double Sf = max_floor(S, 1e-150);
   const double rs = rcp_hd(Sf);
   const double nuh = sqL * uL + sqR * uR, nvh = sqL * vL + sqR * vR, nwh = sqL * wL + sqR * wR;
   const double nHh = sqL * HL + sqR * HR;
   const double uh = nuh * rs, vh = nvh * rs, wh = nwh * rs, Hh = nHh * rs;
   const double q2 = uh * uh + vh * vh + wh * wh;
   const double c2raw = gm1 * (Hh - 0.5 * q2);
-  const double c2 = fmax(c2raw, 1e-30);
+  const double c2 = max_floor(c2raw, 1e-30);
   const double rc2 = rcp_hd(c2);
   const double ch = sqrt_hd(c2);
   const double unh = uh * n[0] + vh * n[1] + wh * n[2];
@@ -351,7 +351,7 @@ FD void viscous_flux_vjp(const double pv[5], double T, const double d[3][4], con
 // T = p / max(rho R, 1e-300) reverse into (rho, p).
 FD void temperature_vjp(const double pv[5], double R, double lT, double& lrho, double& lp) {
   const double den = pv[0] * R;
-  const double dn = fmax(den, 1e-300);
+  const double dn = max_floor(den, 1e-300);
   const double rd = rcp_hd(dn);
   lp += lT * rd;
   lrho -= lT * pv[4] * rd * rd * R * dmax_floor(den, 1e-300);

@@ -52,7 +52,14 @@ struct famrdp_plan {
   std::vector<int64_t> N;
   // device tables
   BlockDesc* d_blocks = nullptr;
-  BlockDesc* h_blocks_pinned = nullptr;  // staging for table uploads
+  // staging ring for block-table uploads: a slot is reused only after the
+  // copy that last read it has completed (its event), so uploads do not
+  // drain the stream
+  static constexpr int NSTAGE = 8;
+  BlockDesc* h_stage[NSTAGE] = {};
+  hipEvent_t stage_ev[NSTAGE] = {};
+  bool stage_used[NSTAGE] = {};
+  int stage_next = 0;
   std::vector<BlockDesc> h_blocks;
   bool blocks_uploaded = false;
   HaloEntry* d_halo = nullptr;
@@ -63,6 +70,7 @@ struct famrdp_plan {
   int64_t bc_maxcells[6] = {0, 0, 0, 0, 0, 0};
   Tile* d_tiles = nullptr;
   int ntiles = 0;
+  double* d_monitor = nullptr;    // per-workgroup partial counts of the state monitor (persistent_grid)
   Tile3* d_tiles3[NUM_TILE_VARIANTS] = {};
   int ntiles3[NUM_TILE_VARIANTS] = {};
   int tile_variant = 0;
@@ -147,38 +155,51 @@ int build_halo_table(famrdp_plan* p, int ncut, const famrdp_cut* cuts, std::vect
   return FAMRDP_OK;
 }
 
-int bind(famrdp_plan* p, double* const* q, const double* const* kxyz, const double* const* vol, hipStream_t st) {
-  if (!q) return fail(FAMRDP_E_INVALID, "q pointer array is NULL");
-  bool changed = !p->blocks_uploaded;
+// Stream-ordered upload of a block table through the staging ring.
+int upload_table(famrdp_plan* p, BlockDesc* dst, const std::vector<BlockDesc>& src, hipStream_t st) {
+  const int s = p->stage_next;
+  p->stage_next = (s + 1) % famrdp_plan::NSTAGE;
+  if (p->stage_used[s]) HIP_TRY(hipEventSynchronize(p->stage_ev[s]));   // that slot's last copy is done
+  std::memcpy(p->h_stage[s], src.data(), sizeof(BlockDesc) * p->nblocks);
+  HIP_TRY(hipMemcpyAsync(dst, p->h_stage[s], sizeof(BlockDesc) * p->nblocks, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipEventRecord(p->stage_ev[s], st));
+  p->stage_used[s] = true;
+  return FAMRDP_OK;
+}
+
+// Point the block tables at the caller's arrays.  q == nullptr leaves the
+// state pointers as they are (the reverse sweep reads its own copy, qh);
+// kxyz / vol == nullptr keep the bound metrics.  Tables are re-uploaded only
+// when a pointer changed.
+int bind(famrdp_plan* p, double* const* q, const double* const* kxyz, const double* const* vol, hipStream_t st,
+         bool need_q = true) {
+  if (need_q && !q) return fail(FAMRDP_E_INVALID, "q pointer array is NULL");
+  bool changed = !p->blocks_uploaded, changed_adj = !p->blocks_uploaded;
   for (int b = 0; b < p->nblocks; ++b) {
     BlockDesc& d = p->h_blocks[b];
-    if (!q[b]) return fail(FAMRDP_E_INVALID, "q[" + std::to_string(b) + "] is NULL");
+    if (q && !q[b]) return fail(FAMRDP_E_INVALID, "q[" + std::to_string(b) + "] is NULL");
     const double* kx = kxyz ? kxyz[b] : d.kxyz;
     const double* vo = vol ? vol[b] : d.vol;
-    if (d.q != q[b] || d.kxyz != kx || d.vol != vo) changed = true;
-    d.q = q[b];
+    if ((q && d.q != q[b]) || d.kxyz != kx || d.vol != vo) changed = true;
+    if (d.kxyz != kx || d.vol != vo) changed_adj = true;
+    if (q) d.q = q[b];
     d.kxyz = kx;
     d.vol = vo;
   }
   if (changed) {
-    // the previous upload (if any) has been consumed in stream order
-    HIP_TRY(hipStreamSynchronize(st));
-    std::memcpy(p->h_blocks_pinned, p->h_blocks.data(), sizeof(BlockDesc) * p->nblocks);
-    HIP_TRY(hipMemcpyAsync(p->d_blocks, p->h_blocks_pinned, sizeof(BlockDesc) * p->nblocks, hipMemcpyHostToDevice, st));
-    if (p->has_adjoint) {
-      for (int b = 0; b < p->nblocks; ++b) {
-        BlockDesc a = p->h_blocks[b];
-        a.q = p->h_blocks_adj[b].qh;
-        p->h_blocks_adj[b] = a;
-      }
-      HIP_TRY(hipStreamSynchronize(st));
-      std::memcpy(p->h_blocks_pinned, p->h_blocks_adj.data(), sizeof(BlockDesc) * p->nblocks);
-      HIP_TRY(hipMemcpyAsync(p->d_blocks_adj, p->h_blocks_pinned, sizeof(BlockDesc) * p->nblocks,
-                             hipMemcpyHostToDevice, st));
-      HIP_TRY(hipStreamSynchronize(st));
-    }
-    p->blocks_uploaded = true;
+    int rc = upload_table(p, p->d_blocks, p->h_blocks, st);
+    if (rc) return rc;
   }
+  if (p->has_adjoint && changed_adj) {
+    for (int b = 0; b < p->nblocks; ++b) {
+      BlockDesc a = p->h_blocks[b];
+      a.q = p->h_blocks_adj[b].qh;
+      p->h_blocks_adj[b] = a;
+    }
+    int rc = upload_table(p, p->d_blocks_adj, p->h_blocks_adj, st);
+    if (rc) return rc;
+  }
+  p->blocks_uploaded = true;
   return FAMRDP_OK;
 }
 
@@ -474,9 +495,16 @@ int famrdp_plan_destroy(famrdp_plan* p) {
   if (p->d_halo) (void)hipFree(p->d_halo);
   if (p->d_halo_stage) (void)hipFree(p->d_halo_stage);
   if (p->d_tiles) (void)hipFree(p->d_tiles);
+  if (p->d_monitor) (void)hipFree(p->d_monitor);
   for (int v = 0; v < NUM_TILE_VARIANTS; ++v) if (p->d_tiles3[v]) (void)hipFree(p->d_tiles3[v]);
   if (p->d_blocks) (void)hipFree(p->d_blocks);
-  if (p->h_blocks_pinned) (void)hipHostFree(p->h_blocks_pinned);
+  for (int s = 0; s < famrdp_plan::NSTAGE; ++s) {
+    if (p->stage_ev[s]) {
+      if (p->stage_used[s]) (void)hipEventSynchronize(p->stage_ev[s]);
+      (void)hipEventDestroy(p->stage_ev[s]);
+    }
+    if (p->h_stage[s]) (void)hipHostFree(p->h_stage[s]);
+  }
   delete p;
   return FAMRDP_OK;
 }
@@ -556,9 +584,12 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     if (hipMalloc(&p->d_blocks_adj, sizeof(BlockDesc) * nblocks) != hipSuccess)
       return cleanup(fail(FAMRDP_E_HIP, "plan_create: adjoint block table allocation failed"));
   }
-  if (hipMalloc(&p->d_blocks, sizeof(BlockDesc) * nblocks) != hipSuccess ||
-      hipHostMalloc(&p->h_blocks_pinned, sizeof(BlockDesc) * nblocks) != hipSuccess)
+  if (hipMalloc(&p->d_blocks, sizeof(BlockDesc) * nblocks) != hipSuccess)
     return cleanup(fail(FAMRDP_E_HIP, "plan_create: block table allocation failed"));
+  for (int s = 0; s < famrdp_plan::NSTAGE; ++s)
+    if (hipHostMalloc(&p->h_stage[s], sizeof(BlockDesc) * nblocks) != hipSuccess ||
+        hipEventCreateWithFlags(&p->stage_ev[s], hipEventDisableTiming) != hipSuccess)
+      return cleanup(fail(FAMRDP_E_HIP, "plan_create: block table staging allocation failed"));
   // halo gather table
   std::vector<HaloEntry> halo;
   rc = build_halo_table(p, ncut, cuts, halo);
@@ -659,6 +690,8 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
   }
   p->persistent_grid = std::min(p->ntiles, ncu * 8);
   p->ncu = ncu;
+  if (hipMalloc(&p->d_monitor, sizeof(double) * p->persistent_grid) != hipSuccess)
+    return cleanup(fail(FAMRDP_E_HIP, "plan_create: monitor buffer allocation failed"));
   *out = p;
   return FAMRDP_OK;
 }
@@ -752,7 +785,9 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
     if (!kxyz[b] || !vol[b] || !lam[b] || !q_start[b] || (q1s && (!q1s[b] || !q2s[b])))
       return fail(FAMRDP_E_INVALID, "step_vjp: NULL pointer for block " + std::to_string(b));
   hipStream_t st = S(stream);
-  int rc = bind(p, q_start, kxyz, vol, st);
+  // metrics only: every reverse kernel reads the adjoint table (state qh),
+  // and q_start is read by the copy below
+  int rc = bind(p, nullptr, kxyz, vol, st, false);
   if (rc) return rc;
   // ---- q0' = BC(halo(q_start)) into qh (BC snapshots kept), then stages 1-2
   rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].qh; }, [&](int b) { return q_start[b]; });
@@ -848,6 +883,18 @@ int famrdp_objective(int32_t kind, const double* q, const double* kxyz, const do
   k_objective<<<parts, 256, 0, S(stream)>>>(kind, q, kxyz, vol, ni, nj, nk, ghost, gamma, grad, work);
   LAUNCH_CHECK();
   k_sum_final<<<1, 256, 0, S(stream)>>>(work, parts, J);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_plan_check_state(famrdp_plan* p, double* const* q, double* out, void* stream) {
+  if (!p || !out) return fail(FAMRDP_E_INVALID, "check_state: NULL plan or output");
+  int rc = bind(p, q, nullptr, nullptr, S(stream));
+  if (rc) return rc;
+  k_monitor_partial<<<p->persistent_grid, 256, 0, S(stream)>>>(p->d_blocks, p->d_tiles, p->ntiles, p->ghost,
+                                                               p->gas.gamma, p->d_monitor);
+  LAUNCH_CHECK();
+  k_sum_parts<<<1, 256, 0, S(stream)>>>(p->d_monitor, p->persistent_grid, out);
   LAUNCH_CHECK();
   return FAMRDP_OK;
 }

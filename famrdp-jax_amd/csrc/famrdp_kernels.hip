@@ -549,6 +549,52 @@ __global__ void __launch_bounds__(256) k_cfl_final(const double* __restrict__ pa
   }
 }
 
+// ---------------------------------------------------------------------------
+// State monitor (Config.nan_check): interior cells of every block of a plan
+// with a non-finite conservative value, rho <= 0 or p <= 0, counted per
+// workgroup over the plan's tile table (no atomics), then summed.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_monitor_partial(const BlockDesc* __restrict__ blocks,
+                                                         const Tile* __restrict__ tiles, int ntiles, int ghost,
+                                                         double gamma, double* __restrict__ part) {
+  __shared__ double sbad[256];
+  double bad = 0.0;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const BlockDesc& b = blocks[tl.block];
+    const int64_t c = (int64_t)tl.start + threadIdx.x;
+    if (c >= b.N) continue;
+    const int k = (int)(c % b.nk), j = (int)((c / b.nk) % b.nj), i = (int)(c / ((int64_t)b.nk * b.nj));
+    if (i < ghost || j < ghost || k < ghost || i >= b.ni - ghost || j >= b.nj - ghost || k >= b.nk - ghost) continue;
+    double pv[5];
+    load_prim(b.q, b.N, c, gamma, pv);
+    const bool ok = isfinite(pv[0]) && isfinite(pv[1]) && isfinite(pv[2]) && isfinite(pv[3]) && isfinite(pv[4]) &&
+                    pv[0] > 0.0 && pv[4] > 0.0;
+    bad += ok ? 0.0 : 1.0;
+  }
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sbad[threadIdx.x] += sbad[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sbad[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) a += part[t];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
 }  // namespace famrdp
 
 // ===========================================================================

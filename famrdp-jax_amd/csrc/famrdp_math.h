@@ -14,6 +14,16 @@ namespace famrdp {
 
 enum Limiter : int { LIM_MINMOD = 0, LIM_VANLEER = 1 };
 
+// NaN semantics.  jnp.maximum / jnp.minimum / jnp.sign return NaN when an
+// operand is NaN, while C's fmax / fmin return the other operand; the floors
+// and the limiter below use these selects so a NaN state goes NaN the way
+// the reference's does.
+//   max_floor(x, c) = jnp.maximum(x, c) for a constant floor c (NaN x -> NaN)
+FD double max_floor(double x, double c) { return x < c ? c : x; }
+//   jnp.minimum(a, b) / jnp.maximum(a, b)
+FD double min_nan(double a, double b) { return a != a ? a : (b != b ? b : fmin(a, b)); }
+FD double max_nan(double a, double b) { return a != a ? a : (b != b ? b : fmax(a, b)); }
+
 // physics/eos.py:12-17
 FD void cons_to_prim(const double q[5], double g, double pv[5]) {
   const double rho = q[0];
@@ -31,19 +41,22 @@ FD void prim_to_cons(const double pv[5], double g, double U[5]) {
   U[4] = p / (g - 1.0) + rho * ke;
 }
 
-// physics/eos.py:24-25; (T/T0)**1.5 evaluated as x*sqrt(x)
+// physics/eos.py:24-25; (T/T0)**1.5 evaluated as x*sqrt(x) (NaN for T < 0,
+// as the reference's power)
 FD double sutherland(double T, double mu0, double T0, double Ts) {
   const double x = T / T0;
   return mu0 * (x * sqrt(x)) * (T0 + Ts) / (T + Ts);
 }
 
-FD double sgn(double a) { return a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : 0.0); }
+// jnp.sign: -1, 0 (signed), +1, NaN for NaN
+FD double sgn(double a) { return a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : a); }
 
 // physics/reconstruct/muscl2.py:5-7: s = sign(a)*max(sign(a)sign(b),0); s*min(|a|,|b|)
+// (NaN in either argument gives NaN)
 FD double minmod(double a, double b) {
   const double sa = sgn(a);
-  const double s = sa * fmax(sa * sgn(b), 0.0);
-  return s * fmin(fabs(a), fabs(b));
+  const double s = sa * max_nan(sa * sgn(b), 0.0);
+  return s * min_nan(fabs(a), fabs(b));
 }
 
 // physics/reconstruct/muscl2.py:10-12
@@ -62,12 +75,16 @@ FD double limit(int lim, double a, double b) {
 // s * min(|a|,|b|) with s in {-1, 0, 1}; a zero's sign never reaches q).
 // As max(min(a,b),0) + min(max(a,b),0): both positive gives min(a,b) + 0,
 // both negative 0 + max(a,b) (the one nearer zero), mixed signs or a zero
-// give 0 -- the reference's value exactly, in four min/max and one add.
+// give 0 -- the reference's value exactly, in four min/max and one add, for
+// non-NaN arguments.  A NaN argument is dropped here (hardware min/max); the
+// face routine that calls it (face_flux_prim) restores the reference's NaN
+// propagation for the whole four-cell stencil instead, at two instructions
+// per face rather than two per limiter call.
 FD double minmod_fast(double a, double b) { return fmax(fmin(a, b), 0.0) + fmin(fmax(a, b), 0.0); }
 
 // flux/roe.py:5-18
 FD void physical_flux(const double U[5], const double n[3], double g, double F[5]) {
-  const double rho = fmax(U[0], 1e-300);
+  const double rho = max_floor(U[0], 1e-300);
   const double u = U[1] / rho, v = U[2] / rho, w = U[3] / rho;
   const double ke = 0.5 * (u * u + v * v + w * w);
   const double p = (g - 1.0) * (U[4] - rho * ke);
@@ -83,7 +100,7 @@ FD void physical_flux(const double U[5], const double n[3], double g, double F[5
 // flux/roe.py:21-85: Roe flux for a unit normal n, Harten fix on lambda1/5.
 FD void roe_flux(const double UL[5], const double UR[5], const double n[3], double g,
                  double eps_entropy, double F[5]) {
-  const double rhoL = fmax(UL[0], 1e-300), rhoR = fmax(UR[0], 1e-300);
+  const double rhoL = max_floor(UL[0], 1e-300), rhoR = max_floor(UR[0], 1e-300);
   const double uL = UL[1] / rhoL, vL = UL[2] / rhoL, wL = UL[3] / rhoL;
   const double uR = UR[1] / rhoR, vR = UR[2] / rhoR, wR = UR[3] / rhoR;
   const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
@@ -93,13 +110,13 @@ FD void roe_flux(const double UL[5], const double UR[5], const double n[3], doub
   const double HL = (UL[4] + pL) / rhoL, HR = (UR[4] + pR) / rhoR;
 
   const double sqL = sqrt(rhoL), sqR = sqrt(rhoR);
-  const double ss = fmax(sqL + sqR, 1e-150);
+  const double ss = max_floor(sqL + sqR, 1e-150);
   const double uh = (sqL * uL + sqR * uR) / ss;
   const double vh = (sqL * vL + sqR * vR) / ss;
   const double wh = (sqL * wL + sqR * wR) / ss;
   const double Hh = (sqL * HL + sqR * HR) / ss;
   const double q2 = uh * uh + vh * vh + wh * wh;
-  const double c2 = fmax((g - 1.0) * (Hh - 0.5 * q2), 1e-30);
+  const double c2 = max_floor((g - 1.0) * (Hh - 0.5 * q2), 1e-30);
   const double ch = sqrt(c2);
   const double unh = uh * n[0] + vh * n[1] + wh * n[2];
   const double rhoh = sqL * sqR;
@@ -184,7 +201,9 @@ DF double rcp(double x) {
 }
 
 // sqrt(x) from the hardware reciprocal square-root seed: one coupled
-// Newton step on (sqrt, 1/(2 sqrt)) and two residual corrections; x >= 0.
+// Newton step on (sqrt, 1/(2 sqrt)) and two residual corrections.  IEEE
+// special cases: +-0 and +inf are returned as they are (the seed gives inf
+// and 0 there), x < 0 and NaN give NaN (the seed is NaN, so g is).
 DF double fsqrt(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   double g = x * y, h = 0.5 * y;
@@ -195,7 +214,8 @@ DF double fsqrt(double x) {
   g = fma(d, h, g);
   d = fma(-g, g, x);
   g = fma(d, h, g);
-  return x > 0.0 ? g : x;
+  // one class test: -0, +0, +inf
+  return __builtin_amdgcn_class(x, (1 << 5) | (1 << 6) | (1 << 9)) ? x : g;
 }
 
 // 1/sqrt(x) for x > 0 from the hardware seed: two coupled Newton steps on
@@ -238,7 +258,7 @@ DF void cons_to_prim_fast(const double q[5], double g, double pv[5]) {
 DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3], double g, double F[5]) {
   const double gm1 = g - 1.0;
   const double rgm1 = rcp(gm1);
-  const double rhoL = fmax(pl[0], 1e-300), rhoR = fmax(pr[0], 1e-300);
+  const double rhoL = max_floor(pl[0], 1e-300), rhoR = max_floor(pr[0], 1e-300);
   const double uL = pl[1], vL = pl[2], wL = pl[3], pL = pl[4];
   const double uR = pr[1], vR = pr[2], wR = pr[3], pR = pr[4];
   const double EL = pL * rgm1 + rhoL * (0.5 * (uL * uL + vL * vL + wL * wL));
@@ -248,13 +268,15 @@ DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3],
   const double HL = (EL + pL) * (isL * isL), HR = (ER + pR) * (isR * isR);
 
   const double sqL = rhoL * isL, sqR = rhoR * isR;
-  const double rs = rcp(fmax(sqL + sqR, 1e-150));
+  // max(sqL + sqR, 1e-150) is sqL + sqR itself: both roots are >= 1e-150
+  // after the 1e-300 floors (and NaN stays NaN), so the floor never acts
+  const double rs = rcp(sqL + sqR);
   const double uh = (sqL * uL + sqR * uR) * rs;
   const double vh = (sqL * vL + sqR * vR) * rs;
   const double wh = (sqL * wL + sqR * wR) * rs;
   const double Hh = (sqL * HL + sqR * HR) * rs;
   const double q2 = uh * uh + vh * vh + wh * wh;
-  const double c2 = fmax(gm1 * (Hh - 0.5 * q2), 1e-30);
+  const double c2 = max_floor(gm1 * (Hh - 0.5 * q2), 1e-30);
   const double isc = rsq_acc(c2);
   const double rc2 = isc * isc;
   const double ch = c2 * isc;
@@ -310,6 +332,17 @@ DF void face_flux_prim(const double pv0[5], const double pv1[5], const double pv
     const double sR = limit_fast<LIM>(pv2[v] - pv1[v], pv3[v] - pv2[v]);
     pL[v] = pv1[v] + 0.5 * sL;
     pR[v] = pv2[v] - 0.5 * sR;
+  }
+  if constexpr (LIM == LIM_MINMOD) {
+    // The reference's minmod is NaN when a slope is, which makes the face
+    // flux NaN whenever any primitive of the four cells is NaN (a NaN in
+    // pL or pR reaches every flux component through the pressure jump).
+    // minmod_fast drops NaN, so the NaN of the outer cells is carried in
+    // here: a cell with any NaN primitive has a NaN pressure (p is computed
+    // from all five conservative variables), and 0 * p is +-0 otherwise.
+    // (van Leer selects 0 for a NaN slope in the reference as well.)
+    pL[4] = fma(0.0, pv0[4], pL[4]);
+    pR[4] = fma(0.0, pv3[4], pR[4]);
   }
   const double nf0 = 0.5 * (k0[0] + k1[0]), nf1 = 0.5 * (k0[1] + k1[1]), nf2 = 0.5 * (k0[2] + k1[2]);
   // |n| + 1e-300 and its reciprocal from one 1/sqrt (|n| = 0: 1e-300, 1e300)
@@ -380,10 +413,10 @@ DF void viscous_flux_fast(const double pv[5], double T, const double d[3][4], co
   G[8] = u * t02 + v * t12 + w * t22 + kappa * gr[3][2];
 }
 
-DF double temperature_fast(const double pv[5], double R) { return pv[4] * rcp(fmax(pv[0] * R, 1e-300)); }
+DF double temperature_fast(const double pv[5], double R) { return pv[4] * rcp(max_floor(pv[0] * R, 1e-300)); }
 
 // T = p / max(rho R, 1e-300)  (rhs/viscous.py:36)
-FD double temperature(const double pv[5], double R) { return pv[4] / fmax(pv[0] * R, 1e-300); }
+FD double temperature(const double pv[5], double R) { return pv[4] / max_floor(pv[0] * R, 1e-300); }
 
 // Tensor component index of G for (row r in 0..3, column j): rows 0..2 are
 // tau_rj, row 3 is E_j.

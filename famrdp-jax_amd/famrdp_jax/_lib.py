@@ -78,6 +78,7 @@ SIGNATURES = {
     "famrdp_halo_exchange": ([vp, ctypes.POINTER(vp), vp], i32),
     "famrdp_apply_bc": ([vp, ctypes.POINTER(vp), vp], i32),
     "famrdp_cfl_dt": ([vp, vp, vp, i32, i32, i32, i32, ctypes.POINTER(Gas), f64, vp, vp, vp], i32),
+    "famrdp_plan_check_state": ([vp, ctypes.POINTER(vp), vp, vp], i32),
 }
 
 E_INVALID, E_HIP, E_UNSUPPORTED, E_KEY = -1, -2, -3, -4

@@ -170,6 +170,21 @@ class Plan:
         arr = lambda x: None if x is None else (L.f64 * self.nblocks)(*[float(v) for v in x])
         L.check(self._lib.famrdp_plan_set_block_params(self._h, arr(mu0), arr(dt_scale)))
 
+    def check_state(self, qs, out):
+        """Enqueue the state monitor: out (device tensor, >= 1 double) gets the
+        number of bad interior cells over all blocks."""
+        L.check(self._lib.famrdp_plan_check_state(self._h, self._ptrs(qs), L.ptr(out), L.stream()))
+
+    def raise_if_bad(self, qs):
+        """Config.nan_check: monitor the state and raise FloatingPointError if
+        any interior cell is non-finite or has rho <= 0 or p <= 0."""
+        torch = L._torch()
+        out = torch.empty(1, dtype=torch.float64, device="cuda")
+        self.check_state(qs, out)
+        nbad = int(out.item())
+        if nbad:
+            raise FloatingPointError(f"nan_check: {nbad} interior cells are non-finite or have rho <= 0 or p <= 0")
+
     def halo_exchange(self, qs):
         L.check(self._lib.famrdp_halo_exchange(self._h, self._ptrs(qs), L.stream()))
 

@@ -48,6 +48,10 @@ class Stepper:
                 self.plan.step(self.qs, self.kxyz, self.vol, dt, batched=batched, direct=direct)
         self.t += dt * nsteps
         self.nstep += nsteps
+        if self.cfg.nan_check and nsteps > 0:
+            # once per call, after its last step (one reduction over the state
+            # and one read of its count)
+            self.plan.raise_if_bad(self.qs)
 
     def autotune(self, dt: float, *, batched: bool = False, reps: int = 3):
         """Pick the fused kernel's tile shape for these blocks (state untouched)."""

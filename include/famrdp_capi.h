@@ -205,6 +205,12 @@ int famrdp_apply_bc(famrdp_plan* plan, double* const* q, void* stream);
 int famrdp_cfl_dt(const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj, int32_t nk,
                   int32_t ghost, const famrdp_gas* gas, double cfl, double* out, double* work, void* stream);
 
+/* Config.nan_check (famrdp_jax/core/types.py:187, carried but never read by
+ * the reference): out (device, 1 double) = number of interior cells, over
+ * every block of the plan, with a non-finite value, rho <= 0 or p <= 0.
+ * Stream-ordered; the caller reads out when it needs the answer. */
+int famrdp_plan_check_state(famrdp_plan* plan, double* const* q, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -129,19 +129,30 @@ def test_reference_grad_case_finite_nonzero_and_matches():
 
 
 def test_central_finite_differences_field_and_mach():
-    """Directional derivative of the wall-pressure J along a random field
-    direction, and dJ/dM_inf of the kinetic-energy J through the farfield
-    state (after 4 steps the farfield barely reaches the wall: the wall J's
-    Mach derivative, ~5e-7, is below what h = 1e-6 resolves), against central
-    differences."""
+    """Directional derivative of the wall-pressure J along a field direction,
+    and dJ/dM_inf of the kinetic-energy J through the farfield state (after 4
+    steps the farfield barely reaches the wall: the wall J's Mach derivative,
+    ~5e-7, is below what h = 1e-6 resolves), against central differences
+    with h = 1e-6, at the 1e-8 bar.
+
+    The step is piecewise smooth: the minmod limiter, |u_n| and the Harten
+    fix switch branches wherever a slope or a wave speed crosses a kink, and
+    this noisy state has such kinks within ~1e-7 relative of many cells, so a
+    random field direction crosses some of them at h = 1e-6 and no
+    derivative matches its difference quotient.  The field direction is
+    therefore the gradient's own (scaled so the largest change of a cell is
+    1e-7 of max|q|), which turns within the smooth piece around q0.  Two
+    roundoff floors are printed: the worst-case eps |J| / (h |dJ|) and the
+    measured one, the change of the difference quotient when h grows by
+    2^-10 (truncation changes by ~h^2 2^-9 there, roundoff does not
+    correlate); the measured floor must be <= 1e-9."""
     print("case: C4-like 12^3 FD checks (h = 1e-6)", flush=True)
     state, cfg, dt = _c4_small(12)
     nsteps = 4
     dev = to_dev(state)
     adj = Adjoint(dev, cfg)
-    rng = np.random.default_rng(31)
-    v = torch.from_numpy(rng.normal(size=adj.q0[0].shape) * np.abs(state.blocks[0].q) * 1e-2).cuda()
     h = 1e-6
+    eps = float(np.finfo(np.float64).eps)
 
     def J_of(kind, q0, qinf=None):
         c = cfg if qinf is None else Config(gas=cfg.gas, scheme=cfg.scheme, bc_params={
@@ -150,19 +161,29 @@ def test_central_finite_differences_field_and_mach():
         out = a.forward(dt, nsteps)
         return a.objective(out, kind)[0]
 
-    _, g, _ = adj.gradient(dt, nsteps, OBJ_WALL_PRESSURE, checkpoint_every=2)
-    fd = (J_of(OBJ_WALL_PRESSURE, adj.q0[0] + h * v) - J_of(OBJ_WALL_PRESSURE, adj.q0[0] - h * v)) / (2 * h)
+    J0, g, _ = adj.gradient(dt, nsteps, OBJ_WALL_PRESSURE, checkpoint_every=2)
+    v = g[0] / g[0].abs().max() * adj.q0[0].abs().max() * 0.05
     ad = float((g[0] * v).sum())
-    print(f"  field: FD {fd:.12e} adjoint {ad:.12e} rel {abs(fd - ad) / abs(ad):.2e}", flush=True)
-    assert abs(fd - ad) <= 1e-6 * abs(ad)
-    _, _, dqinf = adj.gradient(dt, nsteps, OBJ_KINETIC_ENERGY, checkpoint_every=2)
+    fds = [(J_of(OBJ_WALL_PRESSURE, adj.q0[0] + hh * v) - J_of(OBJ_WALL_PRESSURE, adj.q0[0] - hh * v)) / (2 * hh)
+           for hh in (h, h * (1 + 2.0 ** -10))]
+    rel, noise, worst = abs(fds[0] - ad) / abs(ad), abs(fds[0] - fds[1]) / abs(ad), eps * abs(J0) / (h * abs(ad))
+    print(f"  field: FD {fds[0]:.12e} adjoint {ad:.12e} rel {rel:.2e} (floor measured {noise:.2e}, "
+          f"worst case {worst:.2e})", flush=True)
+    assert noise <= 1e-9
+    assert rel <= 1e-8
+    J1, _, dqinf = adj.gradient(dt, nsteps, OBJ_KINETIC_ENERGY, checkpoint_every=2)
     M = 0.8
-    Jp = J_of(OBJ_KINETIC_ENERGY, adj.q0[0], cases.freestream(M + h))
-    Jm = J_of(OBJ_KINETIC_ENERGY, adj.q0[0], cases.freestream(M - h))
-    fdM = (Jp - Jm) / (2 * h)
+    fdM = []
+    for hh in (h, h * (1 + 2.0 ** -10)):
+        Jp = J_of(OBJ_KINETIC_ENERGY, adj.q0[0], cases.freestream(M + hh))
+        Jm = J_of(OBJ_KINETIC_ENERGY, adj.q0[0], cases.freestream(M - hh))
+        fdM.append((Jp - Jm) / (2 * hh))
     adM = mach_derivative(dqinf, M)
-    print(f"  Mach: FD {fdM:.12e} adjoint {adM:.12e} rel {abs(fdM - adM) / abs(adM):.2e}", flush=True)
-    assert abs(fdM - adM) <= 1e-6 * abs(adM)
+    rel, noise, worst = abs(fdM[0] - adM) / abs(adM), abs(fdM[0] - fdM[1]) / abs(adM), eps * abs(J1) / (h * abs(adM))
+    print(f"  Mach: FD {fdM[0]:.12e} adjoint {adM:.12e} rel {rel:.2e} (floor measured {noise:.2e}, "
+          f"worst case {worst:.2e})", flush=True)
+    assert noise <= 1e-9
+    assert rel <= 1e-8
 
 
 def test_kept_stage_states_equal_recomputed():

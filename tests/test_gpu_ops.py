@@ -138,6 +138,17 @@ def test_rk3_step_generic():
     out = rk3_step(q0, lambda q: -q, 0.37)
     dt = 0.37
     np.testing.assert_allclose(out, q0 * (1 - dt + dt * dt / 2 - dt ** 3 / 6), rtol=1e-14)
+    # the reference's keyword name and pytree states (time/rk.py:5-20)
+    a, b = np.linspace(0.5, 1.5, 64), torch.linspace(-2.0, 3.0, 10, dtype=torch.float64, device="cuda")
+    amp = 1 - dt + dt * dt / 2 - dt ** 3 / 6
+    out = rk3_step((a, b), L=lambda q: (-q[0], -2.0 * q[1]), dt=dt)
+    assert isinstance(out, tuple) and isinstance(out[0], np.ndarray) and isinstance(out[1], torch.Tensor)
+    np.testing.assert_allclose(out[0], a * amp, rtol=1e-14)
+    z = -2.0 * dt
+    np.testing.assert_allclose(out[1].cpu().numpy(), b.cpu().numpy() * (1 + z + z * z / 2 + z ** 3 / 6), rtol=1e-14)
+    out = rk3_step({"u": [a], "v": b}, lambda q: {"u": [-q["u"][0]], "v": -q["v"]}, dt)
+    assert set(out) == {"u", "v"} and isinstance(out["u"], list)
+    np.testing.assert_allclose(out["u"][0], a * amp, rtol=1e-14)
 
 
 def _periodic_cfg(shape, gas):

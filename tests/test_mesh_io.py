@@ -105,3 +105,61 @@ def test_file_interfaces_carry_orientation_one():
     from famrdp_jax._plan import cuts_from_topology
     cuts = cuts_from_topology(parse_topology(INP))
     assert cuts and all(c[4] == 1 for c in cuts)
+
+
+def test_aligned_orient_normalisation(tmp_path):
+    """parse_topology(..., aligned_orient=0): the reference fixture's four
+    j-interfaces (same axis, equal tangential extents, identical signed
+    tangential ranges) become orientation 0; a reversed tangential range,
+    unequal tangential extents or different face axes keep orientation 1."""
+    for t in parse_topology(INP, aligned_orient=0):
+        assert t.neighbors[Face.JMIN][2] == 0 and t.neighbors[Face.JMAX][2] == 0
+        assert t.neighbors[Face.JMIN][:2] == (1 - t.block_id, Face.JMAX)
+    txt = open(INP).read()
+    lines = txt.split("\n")
+    # block A's JMIN pair: reverse the neighbour's k range
+    i = lines.index("       1       5        5        5       -3       -1        2")
+    rev = list(lines)
+    rev[i] = "       1       5        5        5       -1       -3        2"
+    p = tmp_path / "rev.inp"
+    p.write_text("\n".join(rev))
+    a, b = parse_topology(p, aligned_orient=0)
+    assert a.neighbors[Face.JMIN][2] == 1 and a.neighbors[Face.JMAX][2] == 0 and b.neighbors[Face.JMIN][2] == 0
+    # unequal tangential extents (block B 6 points along i)
+    ext = list(lines)
+    j = [n for n, ln in enumerate(lines) if ln.strip() == "5      5      3"][1]
+    ext[j] = "     6      5      3"
+    p.write_text("\n".join(ext))
+    assert all(t.neighbors[f][2] == 1 for t in parse_topology(p, aligned_orient=0) for f in (Face.JMIN, Face.JMAX))
+    # a j-face glued to an i-face
+    ax = list(lines)
+    ax[i] = "       5       5        1        5       -3       -1        2"
+    p.write_text("\n".join(ax))
+    a, _ = parse_topology(p, aligned_orient=0)
+    assert a.neighbors[Face.JMIN] == (1, Face.IMAX, 1)
+    assert all(c[4] == 1 for c in __import__("famrdp_jax._plan", fromlist=["x"]).cuts_from_topology(
+        parse_topology(INP)))
+    with pytest.raises(ValueError):
+        parse_topology(INP, aligned_orient=2)
+
+
+def test_file_built_case_steps_on_the_oracle(tmp_path):
+    """The file-built two-block case (mini.inp topology, orientation
+    normalised) runs through the oracle's halo, BCs and three steps and stays
+    finite; without the normalisation the halo rejects it, as the
+    reference's does."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from _filecase import file_case
+    from famrdp_jax import cases
+    from oracle import blocks as OB
+    from oracle import metric as OM
+    state, cfg, dt = file_case(str(tmp_path), OM.compute_metrics)
+    qs = [b.q for b in state.blocks]
+    mets = [(m.kxyz, m.vol) for m in state.metrics]
+    for _ in range(3):
+        qs = OB.step(qs, mets, list(cfg.topology), cfg.bc_params, cases.gas_dict(cfg.gas), dt)
+    assert all(np.isfinite(q).all() for q in qs)
+    assert max(float(np.abs(a - b.q).max()) for a, b in zip(qs, state.blocks)) > 1e-6
+    with pytest.raises(NotImplementedError):
+        OB.halo_exchange([b.q for b in state.blocks], parse_topology(INP), 2)

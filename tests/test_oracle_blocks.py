@@ -196,3 +196,115 @@ def test_cfl_dt_closed_form():
     dt = M.cfl_dt(q, k, np.ones((8, 8, 8)), gas, 0.5)
     lam = (0.5 + 1.0) + 1.0 + 1.0 + (1.4 / 0.72) * 1e-3 * 3
     assert dt == pytest.approx(0.5 / lam, rel=1e-14)
+
+
+# ---- step composition (reference solver/step.py:32-53, time/rk.py:5-20) ------
+#
+# The oracle's ``step`` inlines the Shu-Osher stages over every block after one
+# halo exchange and one BC pass.  Here it is checked against an independent
+# composition of separately pinned pieces: ``halo_exchange`` and ``apply_bc``
+# once, then ``numerics.rk3_step`` (pinned by its exact amplification
+# polynomial) per block over ``rhs_block`` (pinned by the RHS tests).  This
+# fixes the 0.75/0.25 and 1/3, 2/3 coefficients, the halo/BC-once order, and
+# the advancement of ghost cells through the three stages.  The planted
+# mistakes below show that a wrong composition fails the comparison.
+
+from famrdp_jax import cases as CS  # noqa: E402
+
+
+def _metrics_c4(xyz_ext):
+    jac, kxyz, vol = M.compute_metrics(xyz_ext)
+    m = CS.crop_metrics(jac, kxyz, vol)
+    return m.kxyz, m.vol
+
+
+def _composition_cases():
+    st, cfg = CS.case_c1(n=12, seed=3, p_noise=1e-3)
+    c1 = ([b.q for b in st.blocks], [(m.kxyz, m.vol) for m in st.metrics], cfg)
+    q, xyz_ext, cfg4 = CS.case_c4(n=10)
+    c4 = ([q], [_metrics_c4(xyz_ext)], cfg4)
+    st3, cfg3 = CS.case_c3(nb_axis=2, lo=6, hi=10, seed=11, p_noise=1e-3)
+    c3 = ([b.q for b in st3.blocks], [(m.kxyz, m.vol) for m in st3.metrics], cfg3)
+    return {"c1_periodic": c1, "c4_wall_farfield": c4, "c3_2x2x2": c3}
+
+
+_CASES = _composition_cases()
+
+
+def _dt(qs, metrics, gas):
+    return min(M.cfl_dt(q, m[0], m[1], gas, 0.5) for q, m in zip(qs, metrics))
+
+
+def _bc_all(qs, cfg, gas):
+    return [B.apply_bc(q, t, cfg.bc_params, gas["gamma"], gas["R"], 2) for q, t in zip(qs, cfg.topology)]
+
+
+def _composed(qs, metrics, cfg, gas, dt, limiter="minmod"):
+    qs = B.halo_exchange(qs, list(cfg.topology), 2)
+    qs = _bc_all(qs, cfg, gas)
+    return [N.rk3_step(q, lambda x, m=m: N.rhs_block(x, m[0], m[1], gas, limiter), dt) for q, m in zip(qs, metrics)]
+
+
+@pytest.mark.parametrize("name", sorted(_CASES))
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer"])
+def test_step_equals_halo_bc_once_then_rk3(name, limiter):
+    qs, metrics, cfg = _CASES[name]
+    gas = CS.gas_dict(cfg.gas)
+    dt = _dt(qs, metrics, gas)
+    got = B.step(qs, metrics, list(cfg.topology), cfg.bc_params, gas, dt, limiter=limiter)
+    want = _composed(qs, metrics, cfg, gas, dt, limiter)
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    # and the step moved the state by much more than rounding
+    assert max(float(np.abs(g - q).max()) for g, q in zip(got, qs)) > 1e-6
+
+
+def _mutant_refresh_each_stage(qs, metrics, cfg, gas, dt):
+    """Planted mistake: halo + BCs refreshed before every stage."""
+    topos = list(cfg.topology)
+    fix = lambda z: _bc_all(B.halo_exchange(z, topos, 2), cfg, gas)
+    L = lambda z: [N.rhs_block(q, m[0], m[1], gas) for q, m in zip(z, metrics)]
+    q0 = fix(qs)
+    q1 = fix([q + dt * r for q, r in zip(q0, L(q0))])
+    q2 = fix([0.75 * q + 0.25 * (a + dt * r) for q, a, r in zip(q0, q1, L(q1))])
+    return [(1.0 / 3.0) * q + (2.0 / 3.0) * (b + dt * r) for q, b, r in zip(q0, q2, L(q2))]
+
+
+def _mutant_swapped_coefficients(qs, metrics, cfg, gas, dt):
+    """Planted mistake: 0.25/0.75 instead of 0.75/0.25 in stage 2."""
+    qs = _bc_all(B.halo_exchange(qs, list(cfg.topology), 2), cfg, gas)
+    L = lambda z: [N.rhs_block(q, m[0], m[1], gas) for q, m in zip(z, metrics)]
+    q1 = [q + dt * r for q, r in zip(qs, L(qs))]
+    q2 = [0.25 * q + 0.75 * (a + dt * r) for q, a, r in zip(qs, q1, L(q1))]
+    return [(1.0 / 3.0) * q + (2.0 / 3.0) * (b + dt * r) for q, b, r in zip(qs, q2, L(q2))]
+
+
+def _mutant_frozen_ghosts(qs, metrics, cfg, gas, dt):
+    """Planted mistake: ghost cells held at their post-BC values through the
+    stages (only interior cells advance)."""
+    q0 = _bc_all(B.halo_exchange(qs, list(cfg.topology), 2), cfg, gas)
+    L = lambda z: [N.rhs_block(q, m[0], m[1], gas) for q, m in zip(z, metrics)]
+
+    def keep(z):
+        out = []
+        for a, g in zip(z, q0):
+            r = g.copy()
+            r[:, 2:-2, 2:-2, 2:-2] = a[:, 2:-2, 2:-2, 2:-2]
+            out.append(r)
+        return out
+
+    q1 = keep([q + dt * r for q, r in zip(q0, L(q0))])
+    q2 = keep([0.75 * q + 0.25 * (a + dt * r) for q, a, r in zip(q0, q1, L(q1))])
+    return keep([(1.0 / 3.0) * q + (2.0 / 3.0) * (b + dt * r) for q, b, r in zip(q0, q2, L(q2))])
+
+
+@pytest.mark.parametrize("mutant", [_mutant_refresh_each_stage, _mutant_swapped_coefficients, _mutant_frozen_ghosts])
+def test_step_composition_mutants_are_detected(mutant):
+    qs, metrics, cfg = _CASES["c1_periodic"]
+    gas = CS.gas_dict(cfg.gas)
+    dt = _dt(qs, metrics, gas)
+    ref = B.step(qs, metrics, list(cfg.topology), cfg.bc_params, gas, dt)
+    bad = mutant(qs, metrics, cfg, gas, dt)
+    # interior cells only: every mutant changes what the interior receives
+    diff = max(float(np.abs(a - b)[:, 2:-2, 2:-2, 2:-2].max()) for a, b in zip(ref, bad))
+    assert diff > 1e-10, f"{mutant.__name__} not detected ({diff:.3e})"

@@ -260,6 +260,14 @@ hipEvent_t next_event(famrdp_plan* p) {
   return p->ev_pool[p->ev_used++];
 }
 
+// the i-marching variants address a block's arrays with 32-bit byte offsets
+bool variant_fits(const famrdp_plan* p, int v) {
+  if (!is_march_variant(v)) return true;
+  for (int b = 0; b < p->nblocks; ++b)
+    if (!march_fits(p->N[b])) return false;
+  return true;
+}
+
 int launch_stages(famrdp_plan* p, double dt, int flags, hipStream_t st, bool adj = false, int nstages = 3) {
   const bool direct = (flags & FAMRDP_STEP_DIRECT) != 0;
   const BlockDesc* dtab = adj ? p->d_blocks_adj : p->d_blocks;
@@ -323,7 +331,7 @@ extern "C" {
 
 const char* famrdp_last_error(void) { return g_last_error.c_str(); }
 
-const char* famrdp_version(void) { return "famrdp-mi355x 0.7 (gfx950, fp64, fused and split stage kernels, autotuned)"; }
+const char* famrdp_version(void) { return "famrdp-mi355x 0.8 (gfx950, fp64, fused, split and i-marching stage kernels, autotuned)"; }
 
 int famrdp_cons_to_prim(const double* q, double* pv, int64_t n, double gamma, void* stream) {
   if (n < 0 || (n > 0 && (!q || !pv))) return fail(FAMRDP_E_INVALID, "cons_to_prim: bad arguments");
@@ -674,6 +682,12 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     std::vector<Tile3> tiles3;
     for (int b = 0; b < nblocks; ++b) {
       const int32_t* d = &dims[3 * b];
+      if (is_march_variant(v)) {   // i0 = segment index
+        for (int sg = 0; sg < march_nseg(d[0], TI); ++sg)
+          for (int j = 0; j < d[1]; j += TJ)
+            for (int k = 0; k < d[2]; k += TK) tiles3.push_back(Tile3{b, sg, j, k});
+        continue;
+      }
       for (int i = 0; i < d[0]; i += TI)
         for (int j = 0; j < d[1]; j += TJ)
           for (int k = 0; k < d[2]; k += TK) tiles3.push_back(Tile3{b, i, j, k});
@@ -698,6 +712,9 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
 
 int famrdp_plan_set_tile(famrdp_plan* p, int32_t variant) {
   if (!p || variant < 0 || variant >= NUM_TILE_VARIANTS) return fail(FAMRDP_E_INVALID, "set_tile: bad plan or variant");
+  if (!variant_fits(p, variant))
+    return fail(FAMRDP_E_INVALID, "set_tile: variant " + std::to_string(variant) +
+                                      " addresses a block's kxyz with 32-bit offsets (blocks under 59,652,323 cells)");
   p->tile_variant = variant;
   if (p->graph_exec) { (void)hipGraphExecDestroy(p->graph_exec); p->graph_exec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
@@ -724,6 +741,10 @@ int famrdp_plan_autotune(famrdp_plan* p, double* const* q, const double* const* 
   int best = 0;
   float best_ms = 1e30f;
   for (int v = 0; v < NUM_TILE_VARIANTS; ++v) {
+    if (!variant_fits(p, v)) {
+      if (ms_out) ms_out[v] = 1e30;
+      continue;
+    }
     // stages 1 and 2 read q (and q1) and write the q1, q2 workspace only: the
     // state is untouched; stage 2 also reads q0, as stage 3 does
     for (int r = 0; r <= reps; ++r) {

@@ -1269,8 +1269,14 @@ __global__ void __launch_bounds__(NTH) k_stage_split_batched(const BlockDesc* __
   stage_tile_split<TI, TJ, TK, NTH, LIM>(b, tl.i0, tl.j0, tl.k0, qin, b.q, qout, gas, dt, stage);
 }
 
+}  // namespace famrdp
+
+#include "famrdp_march.h"
+
+namespace famrdp {
+
 // Stage-kernel variants the autotuner chooses from (ids 0-4 and 10-13
-// fused, 5-9 split pair).  Fused kernel: (id, TI, TJ, TK, re-staging, waves
+// fused, 5-9 split pair, 14- i-marching).  Fused kernel: (id, TI, TJ, TK, re-staging, waves
 // per SIMD for the register allocation), one thread per cell.  Split pair:
 // (id, TI, TJ, TK, threads of the stage kernel): with one thread per cell an
 // axis's faces take two passes, with a thread per face (NTH1) one.
@@ -1290,16 +1296,34 @@ __global__ void __launch_bounds__(NTH) k_stage_split_batched(const BlockDesc* __
   X(7, 8, 8, 8, 512)              \
   X(8, 2, 8, 16, 256)             \
   X(9, 4, 8, 8, 320)
-constexpr int NUM_TILE_VARIANTS = 14;
+// i-marching kernel: (id, TJ, TK, segment length L, waves per SIMD); its
+// tile table holds (segment start, j0, k0).
+#define FAMRDP_MARCH_VARIANTS(X) \
+  X(14, 16, 16, 32, 2)           \
+  X(15, 16, 16, 64, 2)           \
+  X(16, 8, 32, 32, 2)
+constexpr int NUM_TILE_VARIANTS = 17;
+
+inline bool is_march_variant(int v) {
+  switch (v) {
+#define FAMRDP_IM(id, b, c, l, w) case id: return true;
+    FAMRDP_MARCH_VARIANTS(FAMRDP_IM)
+#undef FAMRDP_IM
+    default: return false;
+  }
+}
 
 inline void tile_dims(int v, int& ti, int& tj, int& tk) {
   switch (v) {
 #define FAMRDP_TD5(id, a, b, c, n) case id: ti = a; tj = b; tk = c; return;
 #define FAMRDP_TD6(id, a, b, c, rs, w) case id: ti = a; tj = b; tk = c; return;
+#define FAMRDP_TDM(id, b, c, l, w) case id: ti = l; tj = b; tk = c; return;
     FAMRDP_TILE_VARIANTS(FAMRDP_TD6)
     FAMRDP_SPLIT_VARIANTS(FAMRDP_TD5)
+    FAMRDP_MARCH_VARIANTS(FAMRDP_TDM)
 #undef FAMRDP_TD5
 #undef FAMRDP_TD6
+#undef FAMRDP_TDM
     default: ti = 4; tj = 4; tk = 16;
   }
 }
@@ -1325,6 +1349,10 @@ inline void launch_stage_lim(int v, const BlockDesc& d, int stage, const Gas& ga
     return;
     FAMRDP_SPLIT_VARIANTS(FAMRDP_SL)
 #undef FAMRDP_SL
+#define FAMRDP_ML(id, b, c, l, w) \
+  case id: k_stage_march<b, c, l, LIM, w><<<nt, b * c, 0, st>>>(d, stage, gas, dt, ntj, ntk); return;
+    FAMRDP_MARCH_VARIANTS(FAMRDP_ML)
+#undef FAMRDP_ML
   }
 }
 
@@ -1353,6 +1381,12 @@ inline void launch_stage_batched_lim(int v, const BlockDesc* blocks, const Tile3
     return;
     FAMRDP_SPLIT_VARIANTS(FAMRDP_SB)
 #undef FAMRDP_SB
+#define FAMRDP_MB(id, b, c, l, w)                                                                      \
+  case id:                                                                                              \
+    k_stage_march_batched<b, c, l, LIM, w><<<ntiles, b * c, 0, st>>>(blocks, tiles, stage, gas, dt);    \
+    return;
+    FAMRDP_MARCH_VARIANTS(FAMRDP_MB)
+#undef FAMRDP_MB
   }
 }
 

@@ -320,6 +320,55 @@ DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3],
                 (Hh * sw + unh * dw + w2 * (0.5 * q2) + w2r * (uh * tx + vh * ty + wh * tz)));
 }
 
+// MUSCL-2 face states of one cell along one axis (physics/reconstruct/
+// muscl2.py:18-37) from the primitives of the cell (pv) and its neighbours
+// (pvm below, pvp above): with the limited slope s = lim(pv - pvm, pvp - pv),
+// the left state of the cell's high face is pv + s/2 and the right state of
+// its low face pv - s/2 -- the slope face_flux_prim evaluates once for each
+// of the two faces, here once.  The minmod form carries the NaN of the cell
+// two away from each face in, as face_flux_prim does.
+template <int LIM>
+DF void muscl_cell(const double pvm[5], const double pv[5], const double pvp[5], double pLhi[5], double pRlo[5]) {
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double s = limit_fast<LIM>(pv[v] - pvm[v], pvp[v] - pv[v]);
+    pLhi[v] = pv[v] + 0.5 * s;
+    pRlo[v] = pv[v] - 0.5 * s;
+  }
+  if constexpr (LIM == LIM_MINMOD) {
+    pLhi[4] = fma(0.0, pvm[4], pLhi[4]);
+    pRlo[4] = fma(0.0, pvp[4], pRlo[4]);
+  }
+}
+
+// The reconstructed states enter the face flux as computed values: every
+// kernel form (faces reconstructed in place, or from states kept from the
+// previous plane) then compiles the same flux code -- the same contractions
+// of products into fused multiply-adds -- so all forms give the same bits.
+DF void face_state_fence(double x[5]) {
+#pragma unroll
+  for (int v = 0; v < 5; ++v) asm volatile("" : "+v"(x[v]));
+}
+
+// Area-weighted Roe flux of one face from its reconstructed left/right
+// primitives and kxyz[d, :] at the cells either side (rhs/invscd.py:53-97).
+DF void face_flux_lr(const double pL[5], const double pR[5], const double k0[3], const double k1[3], double g,
+                     double Fa[5]) {
+  const double nf0 = 0.5 * (k0[0] + k1[0]), nf1 = 0.5 * (k0[1] + k1[1]), nf2 = 0.5 * (k0[2] + k1[2]);
+  // |n| + 1e-300 and its reciprocal from one 1/sqrt (|n| = 0: 1e-300, 1e300)
+  const double nn = nf0 * nf0 + nf1 * nf1 + nf2 * nf2;
+  const double isn = rsq_acc(nn);
+  const bool pos = nn > 0.0;
+  const double nm = (pos ? nn * isn : 0.0) + 1e-300;
+  const double rn = pos ? isn : 1e300;
+  const double nu[3] = {nf0 * rn, nf1 * rn, nf2 * rn};
+  double F[5];
+  roe_flux_prim(pL, pR, nu, g, F);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Fa[v] = F[v] * nm;
+}
+
+
 // One face from the primitives of cells a-1, a, a+1, a+2 and kxyz[d, :] at
 // cells a and a+1 (rhs/invscd.py:53-97), on the stage-kernel forms.
 template <int LIM>
@@ -344,18 +393,9 @@ DF void face_flux_prim(const double pv0[5], const double pv1[5], const double pv
     pL[4] = fma(0.0, pv0[4], pL[4]);
     pR[4] = fma(0.0, pv3[4], pR[4]);
   }
-  const double nf0 = 0.5 * (k0[0] + k1[0]), nf1 = 0.5 * (k0[1] + k1[1]), nf2 = 0.5 * (k0[2] + k1[2]);
-  // |n| + 1e-300 and its reciprocal from one 1/sqrt (|n| = 0: 1e-300, 1e300)
-  const double nn = nf0 * nf0 + nf1 * nf1 + nf2 * nf2;
-  const double isn = rsq_acc(nn);
-  const bool pos = nn > 0.0;
-  const double nm = (pos ? nn * isn : 0.0) + 1e-300;
-  const double rn = pos ? isn : 1e300;
-  const double nu[3] = {nf0 * rn, nf1 * rn, nf2 * rn};
-  double F[5];
-  roe_flux_prim(pL, pR, nu, g, F);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) Fa[v] = F[v] * nm;
+  face_state_fence(pL);
+  face_state_fence(pR);
+  face_flux_lr(pL, pR, k0, k1, g, Fa);
 }
 
 struct Gas {

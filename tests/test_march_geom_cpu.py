@@ -1,0 +1,49 @@
+"""CPU check of the i-marching stage kernel's geometry (famrdp_march_geom.h).
+
+For every workgroup of a block and every thread, each LDS slot the kernel
+writes (primitive plane, viscous-tensor plane, face buffer) lies in its
+buffer and is written once per step; each slot it reads holds what the read
+needs: the periodic (roll) image of the neighbour for the viscous terms
+(rhs/viscous.py:12-16), the unwrapped cell inside the block for the four
+cells of a face (rhs/invscd.py:53-97), the face between the right cells for
+the inviscid difference; and the updates cover every cell of the block
+exactly once over all segments.  Block sizes: partial tiles at both tile
+widths, blocks smaller than a tile, tiny blocks, segment lengths that do and
+do not divide the block.
+
+The header is compiled for the host into a test-only library
+(tests/hostcheck/march_geom_hostcheck.hip); the product never loads it.
+"""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(HERE), "famrdp-jax_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def hc(tmp_path_factory):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    out = str(tmp_path_factory.mktemp("mgc") / "libmgc.so")
+    subprocess.run([hipcc, "--cuda-host-only", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + CSRC,
+                    os.path.join(HERE, "hostcheck", "march_geom_hostcheck.hip"), "-o", out],
+                   check=True, capture_output=True)
+    return ctypes.CDLL(out)
+
+
+@pytest.mark.parametrize("dims", [(20, 20, 20), (36, 36, 36), (23, 37, 41), (70, 18, 45), (5, 5, 5), (6, 7, 9),
+                                  (17, 17, 17), (65, 33, 31), (100, 52, 84), (260, 20, 24)])
+def test_march_geometry(hc, dims):
+    out = np.zeros(3, dtype=np.int64)
+    hc.hc_march_geom_check(*dims, out.ctypes.data_as(ctypes.c_void_p))
+    bad, checked, cover = (int(v) for v in out)
+    assert checked > 0
+    assert bad == 0, f"{bad} of {checked} slot checks failed"
+    assert cover == 0, f"{cover} cells not updated exactly once"

@@ -1300,9 +1300,10 @@ namespace famrdp {
 // tile table holds (segment start, j0, k0).
 #define FAMRDP_MARCH_VARIANTS(X) \
   X(14, 16, 16, 32, 2)           \
-  X(15, 16, 16, 64, 2)           \
-  X(16, 8, 32, 32, 2)
-constexpr int NUM_TILE_VARIANTS = 17;
+  X(15, 8, 32, 20, 2)            \
+  X(16, 8, 32, 32, 2)            \
+  X(17, 8, 32, 52, 2)
+constexpr int NUM_TILE_VARIANTS = 18;
 
 inline bool is_march_variant(int v) {
   switch (v) {

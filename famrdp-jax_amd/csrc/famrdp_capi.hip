@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,38 @@ inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t
 Gas to_gas(const famrdp_gas* g) { return Gas{g->gamma, g->R, g->mu0, g->T0, g->Ts, g->Pr_lam}; }
 
 bool bad_lim(int32_t lim) { return lim != FAMRDP_LIMITER_MINMOD && lim != FAMRDP_LIMITER_VANLEER; }
+
+// Per-workgroup partial results of the CFL reduction (k_cfl_partial), for
+// the entry point that allocates nothing (famrdp_cfl_dt_noalloc): a
+// library-owned array of 2 x CFL_PARTS doubles (16 KiB) on each GPU, held
+// until the process ends and shared by every call on that GPU.  Its address
+// is taken once per device (hipGetSymbolAddress, with that device current)
+// and only that address is handed to kernels and copies.
+constexpr int CFL_PARTS = 1024;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[2 * CFL_PARTS];
+
+namespace {
+
+int cfl_scratch(double** ptr) {
+  static std::mutex mu;
+  static std::vector<double*> per_dev;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
+  if (!per_dev[dev]) {
+    void* a = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !a)
+      return fail(FAMRDP_E_HIP, std::string("cfl_dt_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    per_dev[dev] = static_cast<double*>(a);
+  }
+  *ptr = per_dev[dev];
+  return FAMRDP_OK;
+}
 
 }  // namespace
 
@@ -450,10 +483,27 @@ int famrdp_cfl_dt(const double* q, const double* kxyz, const double* vol, int32_
   if (!q || !kxyz || !vol || !gas || !out || !work || ghost < 0 || ni <= 2 * ghost || nj <= 2 * ghost ||
       nk <= 2 * ghost)
     return fail(FAMRDP_E_INVALID, "cfl_dt: bad arguments");
-  const int parts = 1024;
+  const int parts = CFL_PARTS;
   k_cfl_partial<<<parts, 256, 0, S(stream)>>>(q, kxyz, vol, ni, nj, nk, ghost, to_gas(gas), work, work + parts);
   LAUNCH_CHECK();
   k_cfl_final<<<1, 256, 0, S(stream)>>>(work, work + parts, parts, cfl, out);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
+int famrdp_cfl_dt_noalloc(const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj,
+                          int32_t nk, int32_t ghost, const famrdp_gas* gas, double cfl, double* out, void* stream) {
+  if (!q || !kxyz || !vol || !gas || !out || ghost < 0 || ni <= 2 * ghost || nj <= 2 * ghost || nk <= 2 * ghost)
+    return fail(FAMRDP_E_INVALID, "cfl_dt_noalloc: bad arguments");
+  double* work = nullptr;
+  const int rc = cfl_scratch(&work);
+  if (rc) return rc;
+  // the partials of every workgroup are written before k_cfl_final reads
+  // them: nothing left in the array by an earlier call is read
+  k_cfl_partial<<<CFL_PARTS, 256, 0, S(stream)>>>(q, kxyz, vol, ni, nj, nk, ghost, to_gas(gas), work,
+                                                  work + CFL_PARTS);
+  LAUNCH_CHECK();
+  k_cfl_final<<<1, 256, 0, S(stream)>>>(work, work + CFL_PARTS, CFL_PARTS, cfl, out);
   LAUNCH_CHECK();
   return FAMRDP_OK;
 }

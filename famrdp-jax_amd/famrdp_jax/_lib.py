@@ -80,6 +80,7 @@ SIGNATURES = {
     "famrdp_halo_exchange": ([vp, ctypes.POINTER(vp), vp], i32),
     "famrdp_apply_bc": ([vp, ctypes.POINTER(vp), vp], i32),
     "famrdp_cfl_dt": ([vp, vp, vp, i32, i32, i32, i32, ctypes.POINTER(Gas), f64, vp, vp, vp], i32),
+    "famrdp_cfl_dt_noalloc": ([vp, vp, vp, i32, i32, i32, i32, ctypes.POINTER(Gas), f64, vp, vp], i32),
     "famrdp_plan_check_state": ([vp, ctypes.POINTER(vp), vp, vp], i32),
 }
 

@@ -208,6 +208,18 @@ int famrdp_apply_bc(famrdp_plan* plan, double* const* q, void* stream);
 int famrdp_cfl_dt(const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj, int32_t nk,
                   int32_t ghost, const famrdp_gas* gas, double cfl, double* out, double* work, void* stream);
 
+/* No counterpart in the reference: famrdp_cfl_dt without the work argument,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-workgroup partial results go to a library-owned device array of
+ * 2 x 1024 doubles (16 KiB on each GPU the library runs on, held until the
+ * process ends); its size does not depend on the block, so every block
+ * famrdp_cfl_dt accepts fits.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, two threads) race.  Calls no
+ * allocating routine; same results as famrdp_cfl_dt. */
+int famrdp_cfl_dt_noalloc(const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj,
+                          int32_t nk, int32_t ghost, const famrdp_gas* gas, double cfl, double* out, void* stream);
+
 /* Config.nan_check (famrdp_jax/core/types.py:187, carried but never read by
  * the reference): out (device, 1 double) = number of interior cells, over
  * every block of the plan, with a non-finite value, rho <= 0 or p <= 0.

@@ -205,6 +205,7 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
   march_load5(qin, N, plane(pA) + col, qn);
   if (hs >= 0) march_load5(qin, N, plane(pA) + hcol, qnh);
 
+  double klm[3] = {0, 0, 0};              // kxyz[0, :] of the column at plane s-1
   double pLn[5] = {0, 0, 0, 0, 0};       // left state of the column's next i-face
   double Fm[5] = {0, 0, 0, 0, 0};        // i-face below plane s-1
   double Gm2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // G(s-2) of the column
@@ -229,12 +230,17 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
     march_keep(pn);
     if (s + 2 <= pB + 1) march_load5(qin, N, plane(s + 2) + march_opaque(col), qn);
     double Fi[5] = {0, 0, 0, 0, 0};
+    // the column's metrics of plane s, loaded once: the i-face takes its
+    // right-hand row from them and keeps it as the next step's left-hand row
+    double kk[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) kk[m] = ldx(kx + m * N, c_s);
     {
       double kl[3], kr[3];
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        kl[m] = ldx(kx + m * N, c_m);
-        kr[m] = ldx(kx + m * N, c_s);
+        kl[m] = klm[m];
+        kr[m] = kk[m];
       }
       double pLs[5], pRs[5];
       muscl_cell<LIM>(pm, pc, pn, pLs, pRs);
@@ -244,13 +250,12 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
       march_keep(Fi);
 #pragma unroll
       for (int v = 0; v < 5; ++v) pLn[v] = pLs[v];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) klm[m] = kr[m];
     }
     __builtin_amdgcn_sched_barrier(0);
     double g9[9];
     {
-      double kk[9];
-#pragma unroll
-      for (int m = 0; m < 9; ++m) kk[m] = ldx(kx + m * N, c_s);
       const double um[4] = {pm[1], pm[2], pm[3], pm[5]}, up[4] = {pn[1], pn[2], pn[3], pn[5]};
       march_visc<TJ, TK>(sP, slot, um, up, kk, gb, g9);
       march_keep(g9);

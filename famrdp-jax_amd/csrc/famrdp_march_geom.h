@@ -56,9 +56,10 @@ struct MarchGeom {
   static constexpr int NLDS = 6 * NP + 9 * NGP + 5 * NFB + 6 * NHS + 4 * NHG;
   // Thread roles.  Threads [0, NHG) also stage and compute G at the cross
   // halo cell of the same index; the other halo cells of P are staged by the
-  // threads from T1 (the next wave) on; faces go out in rounds of NT, the
-  // last (partial) round to the threads from TF on.
-  static constexpr int T1 = ((NHG + 63) / 64) * 64;
+  // threads right after them (from T1 = NHG); faces go out in rounds of NT,
+  // the last (partial) round to the threads from TF on -- the last wave(s),
+  // which then stage no halo cell when the halo fits below them.
+  static constexpr int T1 = NHG;
   static constexpr int NFR = (NFB + NT - 1) / NT;
   static constexpr int NFL = NFB - (NFR - 1) * NT;
   static constexpr int TF = NT - ((NFL + 63) / 64) * 64;

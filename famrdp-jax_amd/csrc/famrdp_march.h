@@ -253,7 +253,6 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
 #pragma unroll
       for (int m = 0; m < 3; ++m) klm[m] = kr[m];
     }
-    __builtin_amdgcn_sched_barrier(0);
     double g9[9];
     {
       const double um[4] = {pm[1], pm[2], pm[3], pm[5]}, up[4] = {pn[1], pn[2], pn[3], pn[5]};
@@ -265,7 +264,6 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
       pm[v] = pc[v];
       pc[v] = pn[v];
     }
-    __builtin_amdgcn_sched_barrier(0);
 
     // ---- c. divergence of G(s-1) ------------------------------------------------
     double vis[4] = {0.0, 0.0, 0.0, 0.0};
@@ -290,7 +288,6 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
     }
 #pragma unroll
     for (int m = 0; m < 9; ++m) Gm2[m] = sG[m * M::NGP + gslot];    // G(s-1): G(s'-2) of the next step
-    __builtin_amdgcn_sched_barrier(0);
 
     // ---- d. update of plane s-1 -------------------------------------------------
     if (upd && own) {
@@ -361,7 +358,6 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
         sU[3 * M::NHG + hs] = sP[5 * M::NP + hslot];
       }
     }
-    __builtin_amdgcn_sched_barrier(0);
     if (faces) march_faces<TJ, TK, LIM>(b, j0, k0, op, sP, sF, g, (int)march_opaque((unsigned)t));
     __syncthreads();   // G(s) and the faces of plane s in place; P free
   }

@@ -205,7 +205,7 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
   march_load5(qin, N, plane(pA) + col, qn);
   if (hs >= 0) march_load5(qin, N, plane(pA) + hcol, qnh);
 
-  double klm[3] = {0, 0, 0};              // kxyz[0, :] of the column at plane s-1
+  double kpv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // kxyz of the column at plane s-1
   double pLn[5] = {0, 0, 0, 0, 0};       // left state of the column's next i-face
   double Fm[5] = {0, 0, 0, 0, 0};        // i-face below plane s-1
   double Gm2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // G(s-2) of the column
@@ -231,7 +231,8 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
     if (s + 2 <= pB + 1) march_load5(qin, N, plane(s + 2) + march_opaque(col), qn);
     double Fi[5] = {0, 0, 0, 0, 0};
     // the column's metrics of plane s, loaded once: the i-face takes its
-    // right-hand row from them and keeps it as the next step's left-hand row
+    // right-hand row from them, and they are kept for the next step's
+    // i-face (left-hand row) and divergence of G(s)
     double kk[9];
 #pragma unroll
     for (int m = 0; m < 9; ++m) kk[m] = ldx(kx + m * N, c_s);
@@ -239,7 +240,7 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
       double kl[3], kr[3];
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        kl[m] = klm[m];
+        kl[m] = kpv[m];
         kr[m] = kk[m];
       }
       double pLs[5], pRs[5];
@@ -250,8 +251,6 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
       march_keep(Fi);
 #pragma unroll
       for (int v = 0; v < 5; ++v) pLn[v] = pLs[v];
-#pragma unroll
-      for (int m = 0; m < 3; ++m) klm[m] = kr[m];
     }
     double g9[9];
     {
@@ -268,9 +267,7 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
     // ---- c. divergence of G(s-1) ------------------------------------------------
     double vis[4] = {0.0, 0.0, 0.0, 0.0};
     if (upd && own) {
-      double km[9];
-#pragma unroll
-      for (int m = 0; m < 9; ++m) km[m] = ldx(kx + m * N, c_m);
+      const double* km = kpv;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         double res = 0.0;
@@ -288,6 +285,8 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
     }
 #pragma unroll
     for (int m = 0; m < 9; ++m) Gm2[m] = sG[m * M::NGP + gslot];    // G(s-1): G(s'-2) of the next step
+#pragma unroll
+    for (int m = 0; m < 9; ++m) kpv[m] = kk[m];
 
     // ---- d. update of plane s-1 -------------------------------------------------
     if (upd && own) {

@@ -732,10 +732,14 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     std::vector<Tile3> tiles3;
     for (int b = 0; b < nblocks; ++b) {
       const int32_t* d = &dims[3 * b];
-      if (is_march_variant(v)) {   // i0 = segment index
+      if (is_march_variant(v)) {   // i0 = 2 x segment index + strip bit
+        const MarchTiling mt = march_tiling(TJ, TK, d[1], d[2]);
         for (int sg = 0; sg < march_nseg(d[0], TI); ++sg)
-          for (int j = 0; j < d[1]; j += TJ)
-            for (int k = 0; k < d[2]; k += TK) tiles3.push_back(Tile3{b, sg, j, k});
+          for (int t = 0; t < mt.count(); ++t) {
+            int j0, k0, tr;
+            march_tile_of(mt, TJ, TK, t, j0, k0, tr);
+            tiles3.push_back(Tile3{b, 2 * sg + tr, j0, k0});
+          }
         continue;
       }
       for (int i = 0; i < d[0]; i += TI)

@@ -1297,7 +1297,7 @@ namespace famrdp {
   X(8, 2, 8, 16, 256)             \
   X(9, 4, 8, 8, 320)
 // i-marching kernel: (id, TJ, TK, segment length L, waves per SIMD); its
-// tile table holds (segment start, j0, k0).
+// tile table holds (2 x segment index + strip bit, j0, k0) per march_tiling.
 #define FAMRDP_MARCH_VARIANTS(X) \
   X(14, 16, 16, 32, 2)           \
   X(15, 8, 32, 20, 2)            \
@@ -1350,8 +1350,11 @@ inline void launch_stage_lim(int v, const BlockDesc& d, int stage, const Gas& ga
     return;
     FAMRDP_SPLIT_VARIANTS(FAMRDP_SL)
 #undef FAMRDP_SL
-#define FAMRDP_ML(id, b, c, l, w) \
-  case id: k_stage_march<b, c, l, LIM, w><<<nt, b * c, 0, st>>>(d, stage, gas, dt, ntj, ntk); return;
+#define FAMRDP_ML(id, b, c, l, w)                                                                \
+  case id:                                                                                       \
+    k_stage_march<b, c, l, LIM, w><<<nti * march_tiling(b, c, d.nj, d.nk).count(), b * c, 0, st>>>( \
+        d, stage, gas, dt);                                                                      \
+    return;
     FAMRDP_MARCH_VARIANTS(FAMRDP_ML)
 #undef FAMRDP_ML
   }

@@ -32,7 +32,9 @@
 // the whole march (one plane stride apart from step to step).  Tiles that
 // reach past the block in j or k (partial tiles, small blocks) stage and
 // compute G at the periodic images of those columns, which is what the
-// roll of the viscous terms reads; only columns inside the block write.
+// roll of the viscous terms reads; only columns inside the block write.  A
+// narrow k remainder goes to transposed TK x TJ strip tiles (march_tiling),
+// so that it does not take a row of TJ x TK tiles owning a few columns each.
 // Loads address the arrays with 32-bit byte offsets (uniform base, one
 // vector offset), so a block's kxyz must stay under 4 GiB (march_fits).
 #pragma once
@@ -148,11 +150,10 @@ DF void march_keep(double (&x)[K]) {
 }
 
 template <int TJ, int TK, int LIM>
-__device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0, int pA, int pB,
+__device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, int j0, int k0, int pA, int pB,
                                               const double* __restrict__ qin, const double* q0, double* qout,
                                               const Gas& gas, double dt, int stage) {
   using M = MarchGeom<TJ, TK>;
-  __shared__ double lds[M::NLDS];
   double* sP = lds;                          // [6][NP]  rho u v w p T of plane s
   double* sG = sP + 6 * M::NP;               // [9][NGP] G of plane s-1 (then s)
   double* sF = sG + 9 * M::NGP;              // [5][NFB] j/k faces of plane s-1 (then s)
@@ -362,32 +363,52 @@ __device__ __forceinline__ void march_segment(const BlockDesc& b, int j0, int k0
   }
 }
 
-// Plain launch: grid = segments x (j tiles) x (k tiles), XCD-remapped.
+// One tile's march: TJ x TK columns, or TK x TJ for a strip tile (tr); both
+// shapes use the same LDS layout size.
+template <int TJ, int TK, int LIM>
+__device__ __forceinline__ void march_tile(double* lds, const BlockDesc& b, int tr, int j0, int k0, int pA,
+                                           int pB, const double* __restrict__ qin, double* qout, int stage,
+                                           const Gas& gas, double dt) {
+  static_assert(MarchGeom<TJ, TK>::NLDS == MarchGeom<TK, TJ>::NLDS, "transposed tile LDS");
+  if constexpr (TJ == TK) {
+    march_segment<TJ, TK, LIM>(lds, b, j0, k0, pA, pB, qin, b.q, qout, gas, dt, stage);
+  } else {
+    if (tr) march_segment<TK, TJ, LIM>(lds, b, j0, k0, pA, pB, qin, b.q, qout, gas, dt, stage);
+    else march_segment<TJ, TK, LIM>(lds, b, j0, k0, pA, pB, qin, b.q, qout, gas, dt, stage);
+  }
+}
+
+// Plain launch: grid = segments x march_tiling(...).count(), XCD-remapped.
 // Segments split the block's planes evenly (lengths differ by at most one).
 template <int TJ, int TK, int L, int LIM, int WPE>
 __global__ void __launch_bounds__(TJ * TK) __attribute__((amdgpu_waves_per_eu(WPE)))
-k_stage_march(BlockDesc b, int stage, Gas gas, double dt, int ntj, int ntk) {
+k_stage_march(BlockDesc b, int stage, Gas gas, double dt) {
+  __shared__ double lds[MarchGeom<TJ, TK>::NLDS];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
-  const int tk = w % ntk, tj = (w / ntk) % ntj, seg = w / (ntk * ntj);
+  const MarchTiling mt = march_tiling(TJ, TK, b.nj, b.nk);
+  const int nt = mt.count(), seg = w / nt;
+  int j0, k0, tr;
+  march_tile_of(mt, TJ, TK, w - seg * nt, j0, k0, tr);
   int pA, pB;
   march_segment_bounds(seg, march_nseg(b.ni, L), b.ni, pA, pB);
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-  march_segment<TJ, TK, LIM>(b, tj * TJ, tk * TK, pA, pB, qin, b.q, qout, gas, dt, stage);
+  march_tile<TJ, TK, LIM>(lds, b, tr, j0, k0, pA, pB, qin, qout, stage, gas, dt);
 }
 
-// Batched launch over a tile table whose i0 is the segment index.
+// Batched launch over a tile table whose i0 is 2 x segment index + strip bit.
 template <int TJ, int TK, int L, int LIM, int WPE>
 __global__ void __launch_bounds__(TJ * TK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_stage_march_batched(const BlockDesc* __restrict__ blocks, const Tile3* __restrict__ tiles, int stage, Gas gas,
                       double dt) {
+  __shared__ double lds[MarchGeom<TJ, TK>::NLDS];
   const Tile3 tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
   const BlockDesc& b = blocks[tl.block];
   int pA, pB;
-  march_segment_bounds(tl.i0, march_nseg(b.ni, L), b.ni, pA, pB);
+  march_segment_bounds(tl.i0 >> 1, march_nseg(b.ni, L), b.ni, pA, pB);
   const double* qin = stage == 1 ? b.q : (stage == 2 ? b.q1 : b.q2);
   double* qout = stage == 1 ? b.q1 : (stage == 2 ? b.q2 : b.q);
-  march_segment<TJ, TK, LIM>(b, tl.j0, tl.k0, pA, pB, qin, b.q, qout, gas, dt, stage);
+  march_tile<TJ, TK, LIM>(lds, b, tl.i0 & 1, tl.j0, tl.k0, pA, pB, qin, qout, stage, gas, dt);
 }
 
 }  // namespace famrdp

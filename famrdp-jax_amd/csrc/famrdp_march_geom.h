@@ -43,6 +43,52 @@ __host__ __device__ __forceinline__ void march_segment_bounds(int seg, int nseg,
   pB = (int)((int64_t)(seg + 1) * ni / nseg);
 }
 
+// Tiles of one segment of an nj x nk plane for TJ x TK columns.  The plane is
+// covered by TJ x TK tiles, except that the columns past the last full k
+// tile (nk mod TK of them) go to transposed TK x TJ tiles when that takes
+// fewer workgroups: a narrow remainder then costs a few strip tiles instead
+// of a whole row of TJ x TK tiles that own a few columns each.  Tile t of a
+// segment: the ntj x ntk main tiles first (k fastest), then the sj x sk
+// strip tiles from column k1 on (k fastest).
+struct MarchTiling {
+  int ntj, ntk, k1, sj, sk;
+  __host__ __device__ int count() const { return ntj * ntk + sj * sk; }
+};
+__host__ __device__ inline MarchTiling march_tiling(int TJ, int TK, int nj, int nk) {
+  MarchTiling m;
+  m.ntj = (nj + TJ - 1) / TJ;
+  const int full = nk / TK, r = nk - full * TK;
+  const int sj = (nj + TK - 1) / TK, sk = (r + TJ - 1) / TJ;
+  if (TJ != TK && r > 0 && sj * sk < m.ntj) {
+    m.ntk = full;
+    m.k1 = full * TK;
+    m.sj = sj;
+    m.sk = sk;
+  } else {
+    m.ntk = (nk + TK - 1) / TK;
+    m.k1 = nk;
+    m.sj = 0;
+    m.sk = 0;
+  }
+  return m;
+}
+// origin (j0, k0) of tile t of a segment, and whether it is a strip tile
+// (transposed: TK x TJ columns)
+__host__ __device__ inline void march_tile_of(const MarchTiling& m, int TJ, int TK, int t, int& j0, int& k0,
+                                              int& tr) {
+  const int nmain = m.ntj * m.ntk;
+  if (t < nmain) {
+    j0 = (t / m.ntk) * TJ;
+    k0 = (t % m.ntk) * TK;
+    tr = 0;
+  } else {
+    t -= nmain;
+    j0 = (t / m.sk) * TK;
+    k0 = m.k1 + (t % m.sk) * TJ;
+    tr = 1;
+  }
+}
+
 template <int TJ, int TK>
 struct MarchGeom {
   static constexpr int NT = TJ * TK;                        // threads: one per tile column

@@ -6,7 +6,9 @@
 // written in the same step with the cell (or viscous tensor, or face) the
 // read needs -- the periodic (roll) image for the viscous terms, the
 // unwrapped cell for the faces -- that the faces' cells lie in the block,
-// and that the updates cover every cell of the block exactly once.
+// and that the updates cover every cell of the block exactly once.  The
+// workgroups are the ones the kernel launches (march_tiling: TJ x TK tiles
+// and, for a narrow k remainder, transposed TK x TJ strip tiles).
 // Compiled with --cuda-host-only; never loaded by the product.
 #include <cstdint>
 #include <vector>
@@ -27,12 +29,11 @@ struct Acc {
 
 int wrap(int x, int n) { return ((x % n) + n) % n; }
 
+// thread roles of a tile shape: every halo cell staged once, every face once
 template <int TJ, int TK>
-void check(int ni, int nj, int nk, int L, Acc& acc, std::vector<int>& cover) {
+void check_roles(Acc& acc) {
   using M = MarchGeom<TJ, TK>;
   acc.ok(M::NLDS * 8 <= 160 * 1024);
-  const int ntj = (nj + TJ - 1) / TJ, ntk = (nk + TK - 1) / TK;
-  // thread roles: every halo cell staged once, every face once
   {
     std::vector<int> hcount(M::NHS, 0), fcount(M::NFB, 0);
     for (int t = 0; t < M::NT; ++t) {
@@ -52,9 +53,13 @@ void check(int ni, int nj, int nk, int L, Acc& acc, std::vector<int>& cover) {
     for (int h = 0; h < M::NHS; ++h) acc.ok(hcount[h] == 1);
     for (int f = 0; f < M::NFB; ++f) acc.ok(fcount[f] == 1);
   }
-  for (int tj0 = 0; tj0 < ntj; ++tj0)
-    for (int tk0 = 0; tk0 < ntk; ++tk0) {
-      const int j0 = tj0 * TJ, k0 = tk0 * TK;
+}
+
+// one workgroup: a TJ x TK tile at (j0, k0) over every segment
+template <int TJ, int TK>
+void check_tile(int ni, int nj, int nk, int L, int j0, int k0, Acc& acc, std::vector<int>& cover) {
+  using M = MarchGeom<TJ, TK>;
+    {
       // P: block column (j * nk + k) held by each slot; G plane likewise
       std::vector<int64_t> P(M::NP, -1), G(M::NGP, -1);
       // face buffer: (axis, left cell, other coordinate) of each slot
@@ -150,19 +155,43 @@ void check(int ni, int nj, int nk, int L, Acc& acc, std::vector<int>& cover) {
     }
 }
 
+// every workgroup of a launch with main shape TJ x TK, as march_tiling and
+// march_tile_of lay them out; returns the number of strip tiles
+template <int TJ, int TK>
+int check(int ni, int nj, int nk, int L, Acc& acc, std::vector<int>& cover) {
+  check_roles<TJ, TK>(acc);
+  check_roles<TK, TJ>(acc);
+  acc.ok(MarchGeom<TJ, TK>::NLDS == MarchGeom<TK, TJ>::NLDS);
+  const MarchTiling mt = march_tiling(TJ, TK, nj, nk);
+  int strips = 0;
+  for (int t = 0; t < mt.count(); ++t) {
+    int j0 = 0, k0 = 0, tr = 0;
+    march_tile_of(mt, TJ, TK, t, j0, k0, tr);
+    acc.ok(j0 >= 0 && j0 < nj && k0 >= 0 && k0 < nk);
+    if (tr) {
+      ++strips;
+      check_tile<TK, TJ>(ni, nj, nk, L, j0, k0, acc, cover);
+    } else {
+      check_tile<TJ, TK>(ni, nj, nk, L, j0, k0, acc, cover);
+    }
+  }
+  return strips;
+}
+
 }  // namespace
 
 extern "C" void hc_march_geom_check(int ni, int nj, int nk, int64_t* out) {
   Acc acc;
-  int64_t cells = (int64_t)ni * nj * nk, wrong_cover = 0;
+  int64_t cells = (int64_t)ni * nj * nk, wrong_cover = 0, strips = 0;
   const int shapes[4][3] = {{16, 16, 32}, {8, 32, 20}, {8, 32, 32}, {8, 32, 52}};
   for (const auto& sh : shapes) {
     std::vector<int> cover((size_t)cells, 0);
-    if (sh[0] == 16) check<16, 16>(ni, nj, nk, sh[2], acc, cover);
-    else check<8, 32>(ni, nj, nk, sh[2], acc, cover);
+    if (sh[0] == 16) strips += check<16, 16>(ni, nj, nk, sh[2], acc, cover);
+    else strips += check<8, 32>(ni, nj, nk, sh[2], acc, cover);
     for (int64_t c = 0; c < cells; ++c) wrong_cover += cover[c] != 1;
   }
   out[0] = acc.bad;
   out[1] = acc.checked;
   out[2] = wrong_cover;
+  out[3] = strips;
 }

@@ -185,7 +185,8 @@ def test_tile_variants_single_wrap_blocks_vs_oracle(n):
     conditional wrap, smaller ones a modulo: the C2 grid at 13^3 (17^3
     padded: modulo), 14^3 (18^3: the smallest single-wrap size, where a
     16-cell tile's halo reaches 15 cells past the block end) and 40^3 (44^3:
-    a partial last tile for every tile shape), periodic wrap at both ends of
+    a partial last tile for every tile shape; the 8 x 32 marching variants
+    take the 12-column k remainder in 32 x 8 strip tiles), periodic wrap at both ends of
     each axis; one step per variant against the oracle, fused variants
     bitwise among themselves."""
     print(f"case: C2 grid {n}^3, every tile variant vs oracle, 1 step", flush=True)
@@ -211,11 +212,12 @@ def test_tile_variants_single_wrap_blocks_vs_oracle(n):
 def test_tile_variants_vanleer_vs_oracle():
     """The van Leer limiter (a template instance of every stage kernel of its
     own): every variant, plain and batched, one step against the oracle on
-    the C2 grid at 14^3 (18^3 padded, single wrap) and on the small uneven
-    multi-block case (modulo wrap); fused variants bitwise among themselves."""
+    the C2 grid at 14^3 (18^3 padded, single wrap) and 40^3 (44^3: strip
+    tiles of the marching variants) and on the small uneven multi-block case
+    (modulo wrap); fused variants bitwise among themselves."""
     from famrdp_jax.core.types import Config, SchemeChoice
-    print("case: van Leer, every tile variant vs oracle (C2 grid 14^3, 2x2x2 blocks), 1 step", flush=True)
-    for state, cfg in (_c2_small(14)[:2], _small_multiblock()):
+    print("case: van Leer, every tile variant vs oracle (C2 grid 14^3 and 40^3, 2x2x2 blocks), 1 step", flush=True)
+    for state, cfg in (_c2_small(14)[:2], _c2_small(40)[:2], _small_multiblock()):
         cfg = Config(gas=cfg.gas, scheme=SchemeChoice("muscl2", "roe", 3, "vanleer"), bc_params=cfg.bc_params,
                      topology=cfg.topology, ghost=2)
         dt = 0.5 * min(OM.cfl_dt(b.q, m.kxyz, m.vol, cases.gas_dict(cfg.gas), 0.5)

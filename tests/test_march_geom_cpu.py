@@ -7,9 +7,11 @@ needs: the periodic (roll) image of the neighbour for the viscous terms
 (rhs/viscous.py:12-16), the unwrapped cell inside the block for the four
 cells of a face (rhs/invscd.py:53-97), the face between the right cells for
 the inviscid difference; and the updates cover every cell of the block
-exactly once over all segments.  Block sizes: partial tiles at both tile
-widths, blocks smaller than a tile, tiny blocks, segment lengths that do and
-do not divide the block.
+exactly once over all segments.  The workgroups are the ones the kernel
+launches: TJ x TK tiles and, where the k remainder is narrow, transposed
+TK x TJ strip tiles.  Block sizes: partial tiles at both tile widths, narrow
+k remainders (strip tiles), blocks smaller than a tile, tiny blocks, segment
+lengths that do and do not divide the block.
 
 The header is compiled for the host into a test-only library
 (tests/hostcheck/march_geom_hostcheck.hip); the product never loads it.
@@ -38,12 +40,18 @@ def hc(tmp_path_factory):
     return ctypes.CDLL(out)
 
 
-@pytest.mark.parametrize("dims", [(20, 20, 20), (36, 36, 36), (23, 37, 41), (70, 18, 45), (5, 5, 5), (6, 7, 9),
-                                  (17, 17, 17), (65, 33, 31), (100, 52, 84), (260, 20, 24)])
-def test_march_geometry(hc, dims):
-    out = np.zeros(3, dtype=np.int64)
+# (dims, whether the 8 x 32 launches use strip tiles)
+CASES = [((20, 20, 20), False), ((36, 36, 36), True), ((23, 37, 41), True), ((70, 18, 45), True),
+         ((5, 5, 5), False), ((6, 7, 9), False), ((17, 17, 17), False), ((65, 33, 31), False),
+         ((100, 52, 84), True), ((260, 20, 24), False), ((20, 70, 260), True), ((12, 41, 70), True)]
+
+
+@pytest.mark.parametrize("dims,strips", CASES)
+def test_march_geometry(hc, dims, strips):
+    out = np.zeros(4, dtype=np.int64)
     hc.hc_march_geom_check(*dims, out.ctypes.data_as(ctypes.c_void_p))
-    bad, checked, cover = (int(v) for v in out)
+    bad, checked, cover, nstrip = (int(v) for v in out)
     assert checked > 0
     assert bad == 0, f"{bad} of {checked} slot checks failed"
     assert cover == 0, f"{cover} cells not updated exactly once"
+    assert (nstrip > 0) == strips, nstrip

@@ -216,6 +216,13 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
     const unsigned c_s = op + march_opaque(col), c_m = om + march_opaque(col);
     const bool faces = s >= pA && s < pB;            // plane s is owned: its j/k faces
     const bool upd = s > pA;                         // plane s-1 is owned: its update (s-1 < pB always)
+    // the column's metrics of plane s, loaded once (issued before the barrier,
+    // which covers their latency): the i-face takes its right-hand row from
+    // them, and they are kept for the next step's i-face (left-hand row) and
+    // divergence of G(s)
+    double kk[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) kk[m] = ldx(kx + m * N, c_s);
     // ---- a. P <- plane s ------------------------------------------------------
 #pragma unroll
     for (int v = 0; v < 6; ++v) sP[v * M::NP + slot] = pc[v];
@@ -231,12 +238,6 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
     march_keep(pn);
     if (s + 2 <= pB + 1) march_load5(qin, N, plane(s + 2) + march_opaque(col), qn);
     double Fi[5] = {0, 0, 0, 0, 0};
-    // the column's metrics of plane s, loaded once: the i-face takes its
-    // right-hand row from them, and they are kept for the next step's
-    // i-face (left-hand row) and divergence of G(s)
-    double kk[9];
-#pragma unroll
-    for (int m = 0; m < 9; ++m) kk[m] = ldx(kx + m * N, c_s);
     {
       double kl[3], kr[3];
 #pragma unroll

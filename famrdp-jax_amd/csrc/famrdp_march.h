@@ -254,6 +254,16 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
 #pragma unroll
       for (int v = 0; v < 5; ++v) pLn[v] = pLs[v];
     }
+    // the update operands of plane s-1, issued before G(s) is computed
+    double qs[5] = {0, 0, 0, 0, 0}, qq0[5] = {0, 0, 0, 0, 0}, vl = 1.0;
+    if (upd && own) {
+      vl = ldx(b.vol, c_m);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        qs[v] = ldx(qin + v * N, c_m);
+        qq0[v] = stage == 1 ? 0.0 : ldx(q0 + v * N, c_m);
+      }
+    }
     double g9[9];
     {
       const double um[4] = {pm[1], pm[2], pm[3], pm[5]}, up[4] = {pn[1], pn[2], pn[3], pn[5]};
@@ -292,7 +302,8 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
 
     // ---- d. update of plane s-1 -------------------------------------------------
     if (upd && own) {
-      const double rvol = rcp(ldx(b.vol, c_m) + 1e-300);
+      march_keep(vl);
+      const double rvol = rcp(vl + 1e-300);
       double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       if (s - 1 >= 2 && s - 1 <= ni - 3) {
 #pragma unroll
@@ -314,12 +325,8 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
       rhs[2] = inv[2] + vis[1];
       rhs[3] = inv[3] + vis[2];
       rhs[4] = inv[4] + vis[3];
-      double qs[5], qq0[5];
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        qs[v] = ldx(qin + v * N, c_m);
-        qq0[v] = stage == 1 ? 0.0 : ldx(q0 + v * N, c_m);
-      }
+      march_keep(qs);    // loaded in phase b: combined here as values, as loads
+      march_keep(qq0);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         double* dst = reinterpret_cast<double*>(reinterpret_cast<char*>(qout + v * N) + (c_m << 3));

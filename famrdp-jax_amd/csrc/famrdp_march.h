@@ -270,6 +270,13 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
       march_visc<TJ, TK>(sP, slot, um, up, kk, gb, g9);
       march_keep(g9);
     }
+    // metrics of the G cross-halo cell of plane s, for phase e
+    double kh[9];
+    if (hg) {
+      const unsigned ch = op + march_opaque(hcol);
+#pragma unroll
+      for (int m = 0; m < 9; ++m) kh[m] = ldx(kx + m * N, ch);
+    }
 #pragma unroll
     for (int v = 0; v < 6; ++v) {
       pm[v] = pc[v];
@@ -335,12 +342,6 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) Fm[v] = Fi[v];
-    double kh[9];
-    if (hg) {
-      const unsigned ch = op + march_opaque(hcol);
-#pragma unroll
-      for (int m = 0; m < 9; ++m) kh[m] = ldx(kx + m * N, ch);
-    }
     __syncthreads();   // every read of G(s-1) and of the faces of plane s-1 done
 
     // ---- e. G(s) over G(s-1); G of the cross halo; faces of plane s; next halo --

@@ -1,263 +1,11 @@
-// Adjoint (reverse-mode) kernels of the TVD-RK3 step.
-//
-// One RK stage  q_out = a q0 + b (q_s + dt L(q_s))  is reversed by two
-// gather-form kernels (no atomics, deterministic):
-//   k_vjp_faces_visc
-//                 every cell x: cotangent of the viscous flux tensor G(x)
-//                 from the divergence at its six neighbours, reverse sweep of
-//                 G into d/d(u, v, w, T) and d/d(central differences); then
-//                 every inviscid face whose left cell is x: cotangent of
-//                 F*|n| from the two cells that difference it, Roe + Harten
-//                 reverse sweep, stored as (d/d pL, d/d pR) at x;
-//   k_vjp_cell    every cell y: gathers the face states, the MUSCL slopes of
-//                 y-1, y, y+1 (limiter reverse), the central differences of
-//                 its neighbours and T, maps the primitive cotangent to
-//                 conservative variables and applies the RK coefficients.
-// The halo copy and the BCs are reversed afterwards (BC faces in reverse Face
-// order, the halo through a source-sorted table).  Functions cite the forward
-// code they reverse in famrdp_jax (rhs/invscd.py, rhs/viscous.py,
-// physics/bc/*.py, mesh/halo.py, solver/step.py).
+// Adjoint (reverse-mode) kernels around the stage reverse (famrdp_reverse.hip):
+// the BC faces and the halo copy reversed (BC faces in reverse Face order, the
+// halo through a source-sorted table), the objectives and their gradients, and
+// the state copies of the checkpointing driver.  Functions cite the forward
+// code they reverse in famrdp_jax (physics/bc/*.py, mesh/halo.py).
 #include "famrdp_adjoint.h"
 
 namespace famrdp {
-
-// ---------------------------------------------------------------------------
-// inviscid faces (rhs/invscd.py:53-131 reversed)
-// ---------------------------------------------------------------------------
-template <int LIM>
-__device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double* __restrict__ qs,
-                                               const double* __restrict__ lin, double c_mu, int64_t a, double g) {
-  const int ni = b.ni, nj = b.nj, nk = b.nk;
-  const int64_t N = b.N;
-  const int k = (int)(a % nk), j = (int)((a / nk) % nj), i = (int)(a / ((int64_t)nk * nj));
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double* out = b.fs + (int64_t)(d * 10) * N + a;
-    const int ad = d == 0 ? i : (d == 1 ? j : k);
-    const int nd = d == 0 ? ni : (d == 1 ? nj : nk);
-    const int64_t sd = d == 0 ? (int64_t)nj * nk : (d == 1 ? (int64_t)nk : (int64_t)1);
-    double lpl[5] = {0, 0, 0, 0, 0}, lpr[5] = {0, 0, 0, 0, 0};
-    const bool va = ad >= 2 && ad <= nd - 3, vb = ad + 1 >= 2 && ad + 1 <= nd - 3;
-    if (ad >= 1 && ad <= nd - 3 && (va || vb)) {
-      const int64_t a1 = a + sd;
-      double lFa[5];
-      const double ra = va ? rcp_hd(b.vol[a] + 1e-300) : 0.0, rb = vb ? rcp_hd(b.vol[a1] + 1e-300) : 0.0;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        const double ma = va ? c_mu * lin[v * N + a] : 0.0, mb = vb ? c_mu * lin[v * N + a1] : 0.0;
-        lFa[v] = -ra * ma + rb * mb;
-      }
-      double pv[4][5];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        double qq[5];
-        const int64_t cc = a + (s - 1) * sd;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) qq[v] = qs[v * N + cc];
-        cons_to_prim_fast(qq, g, pv[s]);
-      }
-      double pL[5], pR[5];
-      face_states<LIM>(pv[0], pv[1], pv[2], pv[3], pL, pR);
-      double k0[3], k1[3];
-#pragma unroll
-      for (int m = 0; m < 3; ++m) { k0[m] = b.kxyz[(d * 3 + m) * N + a]; k1[m] = b.kxyz[(d * 3 + m) * N + a1]; }
-      double nu[3], nm;
-      face_normal(k0, k1, nu, nm);
-      double lF[5];
-#pragma unroll
-      for (int v = 0; v < 5; ++v) lF[v] = lFa[v] * nm;
-      roe_flux_prim_vjp(pL, pR, nu, g, lF, lpl, lpr);
-    }
-#pragma unroll
-    for (int v = 0; v < 5; ++v) { out[v * N] = lpl[v]; out[(5 + v) * N] = lpr[v]; }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// viscous flux tensor (rhs/viscous.py:86-108 and 30-84 reversed)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* __restrict__ qs,
-                                              const double* __restrict__ lin, double c_mu, int64_t x,
-                                              const Gas& gas) {
-  const int ni = b.ni, nj = b.nj, nk = b.nk;
-  const int64_t N = b.N;
-  const int k = (int)(x % nk), j = (int)((x / nk) % nj), i = (int)(x / ((int64_t)nk * nj));
-  const int64_t sj = nk, si = (int64_t)nj * nk;
-  const int64_t cm[3] = {x + (wrap_m(i, ni) - i) * si, x + (wrap_m(j, nj) - j) * sj, x + (wrap_m(k, nk) - k)};
-  const int64_t cp[3] = {x + (wrap_p(i, ni) - i) * si, x + (wrap_p(j, nj) - j) * sj, x + (wrap_p(k, nk) - k)};
-  // cotangent of G(x): the divergence at x - e_l uses +G(x)/2, at x + e_l uses -G(x)/2
-  double lG[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int l = 0; l < 3; ++l) {
-    const int64_t ym = cm[l], yp = cp[l];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double mm = c_mu * lin[(r + 1) * N + ym], mp = c_mu * lin[(r + 1) * N + yp];
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-        lG[gidx(r, jj)] += 0.5 * (b.kxyz[(l * 3 + jj) * N + ym] * mm - b.kxyz[(l * 3 + jj) * N + yp] * mp);
-    }
-  }
-  double pv[5], d[3][4];
-  load_prim_fast(qs, N, x, gas.gamma, pv);
-  const double T = temperature_fast(pv, gas.R);
-#pragma unroll
-  for (int l = 0; l < 3; ++l) {
-    double pp[5], pm[5];
-    load_prim_fast(qs, N, cp[l], gas.gamma, pp);
-    load_prim_fast(qs, N, cm[l], gas.gamma, pm);
-    d[l][0] = 0.5 * (pp[1] - pm[1]);
-    d[l][1] = 0.5 * (pp[2] - pm[2]);
-    d[l][2] = 0.5 * (pp[3] - pm[3]);
-    d[l][3] = 0.5 * (temperature_fast(pp, gas.R) - temperature_fast(pm, gas.R));
-  }
-  double kk[9];
-#pragma unroll
-  for (int m = 0; m < 9; ++m) kk[m] = b.kxyz[m * N + x];
-  double lu = 0, lv = 0, lw = 0, lT = 0, ld[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-  viscous_flux_vjp(pv, T, d, kk, gas.gamma, gas.R, b.mu0, gas.T0, gas.Ts, gas.Pr_lam, lG, lu, lv, lw, lT, ld);
-  double* out = b.vs + x;
-  out[0] = lu; out[N] = lv; out[2 * N] = lw; out[3 * N] = lT;
-#pragma unroll
-  for (int l = 0; l < 3; ++l)
-#pragma unroll
-    for (int s = 0; s < 4; ++s) out[(4 + 4 * l + s) * N] = ld[l][s];
-}
-
-// ---------------------------------------------------------------------------
-// per-cell gather, primitive -> conservative, RK coefficients
-//   mode 0: l0 = c0 lin,  lout = c_next lin + g     (stage 3)
-//   mode 1: l0 += c0 lin, lout = c_next lin + g     (stage 2)
-//   mode 2: l0 += lin + g                           (stage 1)
-// ---------------------------------------------------------------------------
-template <int LIM>
-__device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double* __restrict__ qs,
-                                                const double* lin, double* lout, int64_t y, const Gas& gas,
-                                                double c_next, double c0, int mode) {
-  const int ni = b.ni, nj = b.nj, nk = b.nk;
-  const int64_t N = b.N;
-  const int k = (int)(y % nk), j = (int)((y / nk) % nj), i = (int)(y / ((int64_t)nk * nj));
-  double lpv[5] = {0, 0, 0, 0, 0};
-  // ---- inviscid: face states and MUSCL slopes along each axis
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double* fsd = b.fs + (int64_t)(d * 10) * N;
-    const int yd = d == 0 ? i : (d == 1 ? j : k);
-    const int nd = d == 0 ? ni : (d == 1 ? nj : nk);
-    const int64_t sd = d == 0 ? (int64_t)nj * nk : (d == 1 ? (int64_t)nk : (int64_t)1);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      lpv[v] += fsd[v * N + y];                          // left state of face (y, y+1)
-      if (yd >= 1) lpv[v] += fsd[(5 + v) * N + y - sd];  // right state of face (y-1, y)
-    }
-    // slopes s(x) = lim(pv(x)-pv(x-1), pv(x+1)-pv(x)) for x = y-1, y, y+1
-    double pv[5][5];
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-      const int xd = yd + s - 2;
-      if (xd >= 0 && xd < nd) load_prim_fast(qs, N, y + (s - 2) * sd, gas.gamma, pv[s]);
-      else
-#pragma unroll
-        for (int v = 0; v < 5; ++v) pv[s][v] = 0.0;
-    }
-#pragma unroll
-    for (int o = -1; o <= 1; ++o) {
-      const int xd = yd + o;
-      if (xd < 1 || xd > nd - 2) continue;
-      const int64_t xc = y + o * sd;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        const double ls = 0.5 * fsd[v * N + xc] - 0.5 * fsd[(5 + v) * N + xc - sd];
-        const double A = pv[o + 2][v] - pv[o + 1][v], B = pv[o + 3][v] - pv[o + 2][v];
-        double lA, lB;
-        limiter_vjp_t<LIM>(A, B, ls, lA, lB);
-        if (o == 0) lpv[v] += lA - lB;
-        else if (o == 1) lpv[v] -= lA;
-        else lpv[v] += lB;
-      }
-    }
-  }
-  // ---- viscous: own uses plus the central differences of the neighbours
-  const int64_t sj = nk, si = (int64_t)nj * nk;
-  const int64_t cm[3] = {y + (wrap_m(i, ni) - i) * si, y + (wrap_m(j, nj) - j) * sj, y + (wrap_m(k, nk) - k)};
-  const int64_t cp[3] = {y + (wrap_p(i, ni) - i) * si, y + (wrap_p(j, nj) - j) * sj, y + (wrap_p(k, nk) - k)};
-  double ls4[4] = {b.vs[y], b.vs[N + y], b.vs[2 * N + y], b.vs[3 * N + y]};
-#pragma unroll
-  for (int l = 0; l < 3; ++l)
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-      ls4[s] += 0.5 * b.vs[(4 + 4 * l + s) * N + cm[l]] - 0.5 * b.vs[(4 + 4 * l + s) * N + cp[l]];
-  double qy[5], pvy[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) qy[v] = qs[v * N + y];
-  cons_to_prim_fast(qy, gas.gamma, pvy);
-  lpv[1] += ls4[0]; lpv[2] += ls4[1]; lpv[3] += ls4[2];
-  temperature_vjp(pvy, gas.R, ls4[3], lpv[0], lpv[4]);
-  double gq[5] = {0, 0, 0, 0, 0};
-  cons_to_prim_vjp(qy, gas.gamma, lpv, gq);
-  // ---- RK coefficients (solver/step.py:45-53 reversed)
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    const double li = lin[v * N + y];
-    if (mode == 2) {
-      b.l0[v * N + y] += li + gq[v];
-    } else {
-      b.l0[v * N + y] = (mode == 0 ? 0.0 : b.l0[v * N + y]) + c0 * li;
-      lout[v * N + y] = c_next * li + gq[v];
-    }
-  }
-}
-
-// Stage selector for the reverse sweep: which buffers a stage reads/writes.
-//   stage 3: qs = q2, lin = la (cotangent of q_new), lout = lb
-//   stage 2: qs = q1, lin = lb, lout = la
-//   stage 1: qs = qh, lin = la
-__device__ __forceinline__ void vjp_bufs(const BlockDesc& b, int stage, const double*& qs, const double*& lin,
-                                         double*& lout) {
-  qs = stage == 3 ? b.q2 : (stage == 2 ? b.q1 : b.qh);
-  lin = stage == 3 ? b.la : (stage == 2 ? b.lb : b.la);
-  lout = stage == 3 ? b.lb : (stage == 2 ? b.la : nullptr);
-}
-
-__device__ __forceinline__ double stage_cmu(int stage, double dt) {
-  return stage == 3 ? (2.0 / 3.0) * dt : (stage == 2 ? 0.25 * dt : dt);
-}
-
-// The viscous part first: the face part reuses the neighbours' state and
-// metrics it loaded (from registers or cache).  The scheduling barrier keeps
-// the two parts' instructions apart (two waves per SIMD, as the face part
-// alone needs; 4 registers spilled).
-template <int LIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-k_vjp_faces_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int stage, Gas gas, double dt) {
-  const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
-  const BlockDesc& b = blocks[tl.block];
-  const int64_t c = (int64_t)tl.start + threadIdx.x;
-  if (c >= b.N) return;
-  const double *qs, *lin;
-  double* lout;
-  vjp_bufs(b, stage, qs, lin, lout);
-  const double cmu = stage_cmu(stage, dt * b.dt_scale);
-  cell_vjp_visc(b, qs, lin, cmu, c, gas);
-  __builtin_amdgcn_sched_barrier(0);
-  cell_vjp_faces<LIM>(b, qs, lin, cmu, c, gas.gamma);
-}
-
-// at most 128 VGPRs: this memory-bound gather wants four waves per SIMD
-template <int LIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vjp_cell(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
-                                                  int stage, Gas gas) {
-  const Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];   // neighbouring tiles on one XCD (shared L2)
-  const BlockDesc& b = blocks[tl.block];
-  const int64_t c = (int64_t)tl.start + threadIdx.x;
-  if (c >= b.N) return;
-  const double *qs, *lin;
-  double* lout;
-  vjp_bufs(b, stage, qs, lin, lout);
-  const double c_next = stage == 3 ? 2.0 / 3.0 : 0.25;
-  const double c0 = stage == 3 ? 1.0 / 3.0 : 0.75;
-  cell_vjp_gather<LIM>(b, qs, lin, lout, c, gas, c_next, c0, stage == 3 ? 0 : (stage == 2 ? 1 : 2));
-}
 
 // ---------------------------------------------------------------------------
 // BC faces reversed (physics/bc/*.py): the ghost cotangent moves to the inner

@@ -12,6 +12,7 @@
 
 #include "famrdp_kernels.hip"
 #include "famrdp_adjoint.hip"
+#include "famrdp_reverse.h"
 #include "../../include/famrdp_capi.h"
 
 using namespace famrdp;
@@ -885,15 +886,7 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
                    [&](int b) { return static_cast<const double*>(lam[b]); });
   if (rc) return rc;
   for (int stage = 3; stage >= 1; --stage) {
-    if (p->limiter == LIM_VANLEER)
-      k_vjp_faces_visc<LIM_VANLEER><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, dt);
-    else
-      k_vjp_faces_visc<LIM_MINMOD><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas, dt);
-    LAUNCH_CHECK();
-    if (p->limiter == LIM_VANLEER)
-      k_vjp_cell<LIM_VANLEER><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas);
-    else
-      k_vjp_cell<LIM_MINMOD><<<p->ntiles, 256, 0, st>>>(p->d_blocks_adj, p->d_tiles, stage, p->gas);
+    launch_stage_vjp(p->d_blocks_adj, p->d_tiles, p->ntiles, stage, p->gas, dt, p->limiter, st);
     LAUNCH_CHECK();
   }
   // ---- BCs reversed (Face order backwards), then the halo copy

@@ -59,4 +59,35 @@ struct Tile {
   int32_t block, start;
 };
 
+// periodic neighbours along an axis of n cells
+__device__ __forceinline__ int wrap_m(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__device__ __forceinline__ int wrap_p(int i, int n) { return i == n - 1 ? 0 : i + 1; }
+
+// primitives of cell c of a (5, N) state
+__device__ __forceinline__ void load_prim(const double* __restrict__ q, int64_t N, int64_t c, double g,
+                                          double pv[5]) {
+  double qc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qc[v] = q[v * N + c];
+  cons_to_prim(qc, g, pv);
+}
+
+__device__ __forceinline__ void load_prim_fast(const double* __restrict__ q, int64_t N, int64_t c, double g,
+                                               double pv[5]) {
+  double qc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qc[v] = q[v * N + c];
+  cons_to_prim_fast(qc, g, pv);
+}
+
+// XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so map
+// consecutive ids of one XCD onto consecutive tiles (neighbouring tiles share
+// halo lines in that XCD's L2).
+__device__ __forceinline__ int xcd_remap(int wg, int nwg) {
+  constexpr int X = 8;
+  const int q = nwg / X, r = nwg % X;
+  const int xcd = wg % X, local = wg / X;
+  return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
 }  // namespace famrdp

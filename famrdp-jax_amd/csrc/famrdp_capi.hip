@@ -594,6 +594,11 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
       return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": each padded size must be >= 2*ghost"));
     if ((int64_t)d[0] * d[1] * d[2] > INT32_MAX)   // tile kernels index cells in 32 bits
       return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": 2^31 or more cells"));
+    // the reverse kernels address cells with 32-bit byte offsets; an adjoint
+    // plan allocates 76 doubles per cell (stage and adjoint buffers), so such a
+    // block would need over 300 GB of them anyway
+    if ((plan_flags & FAMRDP_PLAN_ADJOINT) && (int64_t)d[0] * d[1] * d[2] >= (int64_t(1) << 29))
+      return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": 2^29 or more cells in an adjoint plan"));
     p->N.push_back((int64_t)d[0] * d[1] * d[2]);
   }
   // workspace and block table

@@ -59,6 +59,14 @@ struct Tile {
   int32_t block, start;
 };
 
+// Doubles in global memory.  Pointers read from a device table (the batched
+// launches' block descriptors) are generic to the compiler, which then loads
+// through them with flat instructions (64-bit vector addresses, counted as
+// LDS accesses as well); a cast to these types gives global loads and stores
+// with a scalar base and a 32-bit vector offset.
+typedef __attribute__((address_space(1))) const double gmem_cdouble;
+typedef __attribute__((address_space(1))) double gmem_double;
+
 // periodic neighbours along an axis of n cells
 __device__ __forceinline__ int wrap_m(int i, int n) { return i == 0 ? n - 1 : i - 1; }
 __device__ __forceinline__ int wrap_p(int i, int n) { return i == n - 1 ? 0 : i + 1; }

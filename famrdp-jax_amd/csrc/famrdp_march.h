@@ -50,10 +50,14 @@ DF void march_prim(const double q[5], double g, double R, double pv[6]) {
   pv[5] = temperature_fast(pv, R);
 }
 
-// a[i] through a 32-bit byte offset from a uniform base (i < 2^29)
+// a[i] through a 32-bit byte offset from a uniform base (i < 2^29), as a
+// global load
 DF double ldx(const double* __restrict__ a, unsigned i) {
-  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(a) + (i << 3));
+  return *(gmem_cdouble*)(reinterpret_cast<const char*>(a) + (i << 3));
 }
+
+// a[i] = x, the same way
+DF void stx(double* a, unsigned i, double x) { *(gmem_double*)(reinterpret_cast<char*>(a) + (i << 3)) = x; }
 
 DF void march_load5(const double* __restrict__ a, int64_t N, unsigned off, double x[5]) {
 #pragma unroll
@@ -336,8 +340,7 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
       march_keep(qq0);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        double* dst = reinterpret_cast<double*>(reinterpret_cast<char*>(qout + v * N) + (c_m << 3));
-        *dst = rk_combine(stage, qq0[v], qs[v], dt, rhs[v]);
+        stx(qout + v * N, c_m, rk_combine(stage, qq0[v], qs[v], dt, rhs[v]));
       }
     }
 #pragma unroll

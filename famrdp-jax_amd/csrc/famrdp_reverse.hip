@@ -24,46 +24,62 @@
 
 namespace famrdp {
 
+// Cells are addressed with 32-bit element indices (an adjoint plan's blocks
+// hold fewer than 2^29 cells: famrdp_plan_create refuses larger ones, whose 76
+// doubles per cell of stage and adjoint buffers would exceed 300 GB), loads and stores through a 32-bit byte offset from
+// a base that is the same for the whole workgroup -- the base stays in scalar
+// registers and a cell costs one 32-bit offset, however many arrays it reads
+// (global loads and stores: the pointers come from the device block table).
+__device__ __forceinline__ double rd(const double* __restrict__ a, unsigned i) {
+  return *(gmem_cdouble*)(reinterpret_cast<const char*>(a) + (i << 3));
+}
+__device__ __forceinline__ gmem_double& at(double* a, unsigned i) {
+  return *(gmem_double*)(reinterpret_cast<char*>(a) + (i << 3));
+}
+
 // ---------------------------------------------------------------------------
 // inviscid faces (rhs/invscd.py:53-131 reversed)
 // ---------------------------------------------------------------------------
 template <int LIM>
 __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double* __restrict__ qs,
-                                               const double* __restrict__ lin, double c_mu, int64_t a, double g) {
+                                               const double* __restrict__ lin, double c_mu, unsigned a, double g) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
-  const int k = (int)(a % nk), j = (int)((a / nk) % nj), i = (int)(a / ((int64_t)nk * nj));
+  const unsigned njk = (unsigned)nj * (unsigned)nk;
+  const int k = (int)(a % (unsigned)nk), j = (int)((a / (unsigned)nk) % (unsigned)nj), i = (int)(a / njk);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double* out = b.fs + (int64_t)(d * 10) * N + a;
     const int ad = d == 0 ? i : (d == 1 ? j : k);
     const int nd = d == 0 ? ni : (d == 1 ? nj : nk);
-    const int64_t sd = d == 0 ? (int64_t)nj * nk : (d == 1 ? (int64_t)nk : (int64_t)1);
+    const unsigned sd = d == 0 ? njk : (d == 1 ? (unsigned)nk : 1u);
     double lpl[5] = {0, 0, 0, 0, 0}, lpr[5] = {0, 0, 0, 0, 0};
     const bool va = ad >= 2 && ad <= nd - 3, vb = ad + 1 >= 2 && ad + 1 <= nd - 3;
     if (ad >= 1 && ad <= nd - 3 && (va || vb)) {
-      const int64_t a1 = a + sd;
+      const unsigned a1 = a + sd;
       double lFa[5];
-      const double ra = va ? rcp_hd(b.vol[a] + 1e-300) : 0.0, rb = vb ? rcp_hd(b.vol[a1] + 1e-300) : 0.0;
+      const double ra = va ? rcp_hd(rd(b.vol, a) + 1e-300) : 0.0, rb = vb ? rcp_hd(rd(b.vol, a1) + 1e-300) : 0.0;
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double ma = va ? c_mu * lin[v * N + a] : 0.0, mb = vb ? c_mu * lin[v * N + a1] : 0.0;
+        const double ma = va ? c_mu * rd(lin + v * N, a) : 0.0, mb = vb ? c_mu * rd(lin + v * N, a1) : 0.0;
         lFa[v] = -ra * ma + rb * mb;
       }
       double pv[4][5];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         double qq[5];
-        const int64_t cc = a + (s - 1) * sd;
+        const unsigned cc = a + (unsigned)(s - 1) * sd;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) qq[v] = qs[v * N + cc];
+        for (int v = 0; v < 5; ++v) qq[v] = rd(qs + v * N, cc);
         cons_to_prim_fast(qq, g, pv[s]);
       }
       double pL[5], pR[5];
       face_states<LIM>(pv[0], pv[1], pv[2], pv[3], pL, pR);
       double k0[3], k1[3];
 #pragma unroll
-      for (int m = 0; m < 3; ++m) { k0[m] = b.kxyz[(d * 3 + m) * N + a]; k1[m] = b.kxyz[(d * 3 + m) * N + a1]; }
+      for (int m = 0; m < 3; ++m) {
+        k0[m] = rd(b.kxyz + (d * 3 + m) * N, a);
+        k1[m] = rd(b.kxyz + (d * 3 + m) * N, a1);
+      }
       double nu[3], nm;
       face_normal(k0, k1, nu, nm);
       double lF[5];
@@ -72,43 +88,55 @@ __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double*
       roe_flux_prim_vjp(pL, pR, nu, g, lF, lpl, lpr);
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) { out[v * N] = lpl[v]; out[(5 + v) * N] = lpr[v]; }
+    for (int v = 0; v < 5; ++v) {
+      at(b.fs + (d * 10 + v) * N, a) = lpl[v];
+      at(b.fs + (d * 10 + 5 + v) * N, a) = lpr[v];
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // viscous flux tensor (rhs/viscous.py:86-108 and 30-84 reversed)
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void rd_prim(const double* __restrict__ q, int64_t N, unsigned c, double g, double pv[5]) {
+  double qc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qc[v] = rd(q + v * N, c);
+  cons_to_prim_fast(qc, g, pv);
+}
+
 __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* __restrict__ qs,
-                                              const double* __restrict__ lin, double c_mu, int64_t x,
+                                              const double* __restrict__ lin, double c_mu, unsigned x,
                                               const Gas& gas) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
-  const int k = (int)(x % nk), j = (int)((x / nk) % nj), i = (int)(x / ((int64_t)nk * nj));
-  const int64_t sj = nk, si = (int64_t)nj * nk;
-  const int64_t cm[3] = {x + (wrap_m(i, ni) - i) * si, x + (wrap_m(j, nj) - j) * sj, x + (wrap_m(k, nk) - k)};
-  const int64_t cp[3] = {x + (wrap_p(i, ni) - i) * si, x + (wrap_p(j, nj) - j) * sj, x + (wrap_p(k, nk) - k)};
+  const unsigned sj = (unsigned)nk, si = (unsigned)nj * (unsigned)nk;
+  const int k = (int)(x % sj), j = (int)((x / sj) % (unsigned)nj), i = (int)(x / si);
+  const unsigned cm[3] = {x + (unsigned)(wrap_m(i, ni) - i) * si, x + (unsigned)(wrap_m(j, nj) - j) * sj,
+                          x + (unsigned)(wrap_m(k, nk) - k)};
+  const unsigned cp[3] = {x + (unsigned)(wrap_p(i, ni) - i) * si, x + (unsigned)(wrap_p(j, nj) - j) * sj,
+                          x + (unsigned)(wrap_p(k, nk) - k)};
   // cotangent of G(x): the divergence at x - e_l uses +G(x)/2, at x + e_l uses -G(x)/2
   double lG[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
-    const int64_t ym = cm[l], yp = cp[l];
+    const unsigned ym = cm[l], yp = cp[l];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double mm = c_mu * lin[(r + 1) * N + ym], mp = c_mu * lin[(r + 1) * N + yp];
+      const double mm = c_mu * rd(lin + (r + 1) * N, ym), mp = c_mu * rd(lin + (r + 1) * N, yp);
 #pragma unroll
       for (int jj = 0; jj < 3; ++jj)
-        lG[gidx(r, jj)] += 0.5 * (b.kxyz[(l * 3 + jj) * N + ym] * mm - b.kxyz[(l * 3 + jj) * N + yp] * mp);
+        lG[gidx(r, jj)] += 0.5 * (rd(b.kxyz + (l * 3 + jj) * N, ym) * mm - rd(b.kxyz + (l * 3 + jj) * N, yp) * mp);
     }
   }
   double pv[5], d[3][4];
-  load_prim_fast(qs, N, x, gas.gamma, pv);
+  rd_prim(qs, N, x, gas.gamma, pv);
   const double T = temperature_fast(pv, gas.R);
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     double pp[5], pm[5];
-    load_prim_fast(qs, N, cp[l], gas.gamma, pp);
-    load_prim_fast(qs, N, cm[l], gas.gamma, pm);
+    rd_prim(qs, N, cp[l], gas.gamma, pp);
+    rd_prim(qs, N, cm[l], gas.gamma, pm);
     d[l][0] = 0.5 * (pp[1] - pm[1]);
     d[l][1] = 0.5 * (pp[2] - pm[2]);
     d[l][2] = 0.5 * (pp[3] - pm[3]);
@@ -116,15 +144,17 @@ __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* 
   }
   double kk[9];
 #pragma unroll
-  for (int m = 0; m < 9; ++m) kk[m] = b.kxyz[m * N + x];
+  for (int m = 0; m < 9; ++m) kk[m] = rd(b.kxyz + m * N, x);
   double lu = 0, lv = 0, lw = 0, lT = 0, ld[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
   viscous_flux_vjp(pv, T, d, kk, gas.gamma, gas.R, b.mu0, gas.T0, gas.Ts, gas.Pr_lam, lG, lu, lv, lw, lT, ld);
-  double* out = b.vs + x;
-  out[0] = lu; out[N] = lv; out[2 * N] = lw; out[3 * N] = lT;
+  at(b.vs, x) = lu;
+  at(b.vs + N, x) = lv;
+  at(b.vs + 2 * N, x) = lw;
+  at(b.vs + 3 * N, x) = lT;
 #pragma unroll
   for (int l = 0; l < 3; ++l)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) out[(4 + 4 * l + s) * N] = ld[l][s];
+    for (int s = 0; s < 4; ++s) at(b.vs + (4 + 4 * l + s) * N, x) = ld[l][s];
 }
 
 // ---------------------------------------------------------------------------
@@ -135,11 +165,12 @@ __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* 
 // ---------------------------------------------------------------------------
 template <int LIM>
 __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double* __restrict__ qs,
-                                                const double* lin, double* lout, int64_t y, const Gas& gas,
+                                                const double* lin, double* lout, unsigned y, const Gas& gas,
                                                 double c_next, double c0, int mode) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
-  const int k = (int)(y % nk), j = (int)((y / nk) % nj), i = (int)(y / ((int64_t)nk * nj));
+  const unsigned sj = (unsigned)nk, si = (unsigned)nj * (unsigned)nk;
+  const int k = (int)(y % sj), j = (int)((y / sj) % (unsigned)nj), i = (int)(y / si);
   double lpv[5] = {0, 0, 0, 0, 0};
   // ---- inviscid: face states and MUSCL slopes along each axis
 #pragma unroll
@@ -147,18 +178,18 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
     const double* fsd = b.fs + (int64_t)(d * 10) * N;
     const int yd = d == 0 ? i : (d == 1 ? j : k);
     const int nd = d == 0 ? ni : (d == 1 ? nj : nk);
-    const int64_t sd = d == 0 ? (int64_t)nj * nk : (d == 1 ? (int64_t)nk : (int64_t)1);
+    const unsigned sd = d == 0 ? si : (d == 1 ? sj : 1u);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      lpv[v] += fsd[v * N + y];                          // left state of face (y, y+1)
-      if (yd >= 1) lpv[v] += fsd[(5 + v) * N + y - sd];  // right state of face (y-1, y)
+      lpv[v] += rd(fsd + v * N, y);                             // left state of face (y, y+1)
+      if (yd >= 1) lpv[v] += rd(fsd + (5 + v) * N, y - sd);     // right state of face (y-1, y)
     }
     // slopes s(x) = lim(pv(x)-pv(x-1), pv(x+1)-pv(x)) for x = y-1, y, y+1
     double pv[5][5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
       const int xd = yd + s - 2;
-      if (xd >= 0 && xd < nd) load_prim_fast(qs, N, y + (s - 2) * sd, gas.gamma, pv[s]);
+      if (xd >= 0 && xd < nd) rd_prim(qs, N, y + (unsigned)(s - 2) * sd, gas.gamma, pv[s]);
       else
 #pragma unroll
         for (int v = 0; v < 5; ++v) pv[s][v] = 0.0;
@@ -167,10 +198,10 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
     for (int o = -1; o <= 1; ++o) {
       const int xd = yd + o;
       if (xd < 1 || xd > nd - 2) continue;
-      const int64_t xc = y + o * sd;
+      const unsigned xc = y + (unsigned)o * sd;
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double ls = 0.5 * fsd[v * N + xc] - 0.5 * fsd[(5 + v) * N + xc - sd];
+        const double ls = 0.5 * rd(fsd + v * N, xc) - 0.5 * rd(fsd + (5 + v) * N, xc - sd);
         const double A = pv[o + 2][v] - pv[o + 1][v], B = pv[o + 3][v] - pv[o + 2][v];
         double lA, lB;
         limiter_vjp_t<LIM>(A, B, ls, lA, lB);
@@ -181,18 +212,19 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
     }
   }
   // ---- viscous: own uses plus the central differences of the neighbours
-  const int64_t sj = nk, si = (int64_t)nj * nk;
-  const int64_t cm[3] = {y + (wrap_m(i, ni) - i) * si, y + (wrap_m(j, nj) - j) * sj, y + (wrap_m(k, nk) - k)};
-  const int64_t cp[3] = {y + (wrap_p(i, ni) - i) * si, y + (wrap_p(j, nj) - j) * sj, y + (wrap_p(k, nk) - k)};
-  double ls4[4] = {b.vs[y], b.vs[N + y], b.vs[2 * N + y], b.vs[3 * N + y]};
+  const unsigned cm[3] = {y + (unsigned)(wrap_m(i, ni) - i) * si, y + (unsigned)(wrap_m(j, nj) - j) * sj,
+                          y + (unsigned)(wrap_m(k, nk) - k)};
+  const unsigned cp[3] = {y + (unsigned)(wrap_p(i, ni) - i) * si, y + (unsigned)(wrap_p(j, nj) - j) * sj,
+                          y + (unsigned)(wrap_p(k, nk) - k)};
+  double ls4[4] = {rd(b.vs, y), rd(b.vs + N, y), rd(b.vs + 2 * N, y), rd(b.vs + 3 * N, y)};
 #pragma unroll
   for (int l = 0; l < 3; ++l)
 #pragma unroll
     for (int s = 0; s < 4; ++s)
-      ls4[s] += 0.5 * b.vs[(4 + 4 * l + s) * N + cm[l]] - 0.5 * b.vs[(4 + 4 * l + s) * N + cp[l]];
+      ls4[s] += 0.5 * rd(b.vs + (4 + 4 * l + s) * N, cm[l]) - 0.5 * rd(b.vs + (4 + 4 * l + s) * N, cp[l]);
   double qy[5], pvy[5];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) qy[v] = qs[v * N + y];
+  for (int v = 0; v < 5; ++v) qy[v] = rd(qs + v * N, y);
   cons_to_prim_fast(qy, gas.gamma, pvy);
   lpv[1] += ls4[0]; lpv[2] += ls4[1]; lpv[3] += ls4[2];
   temperature_vjp(pvy, gas.R, ls4[3], lpv[0], lpv[4]);
@@ -201,12 +233,13 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
   // ---- RK coefficients (solver/step.py:45-53 reversed)
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    const double li = lin[v * N + y];
+    const double li = rd(lin + v * N, y);
+    gmem_double& l0 = at(b.l0 + v * N, y);
     if (mode == 2) {
-      b.l0[v * N + y] += li + gq[v];
+      l0 += li + gq[v];
     } else {
-      b.l0[v * N + y] = (mode == 0 ? 0.0 : b.l0[v * N + y]) + c0 * li;
-      lout[v * N + y] = c_next * li + gq[v];
+      l0 = (mode == 0 ? 0.0 : l0) + c0 * li;
+      at(lout + v * N, y) = c_next * li + gq[v];
     }
   }
 }
@@ -241,9 +274,9 @@ k_vjp_faces_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ 
   double* lout;
   vjp_bufs(b, stage, qs, lin, lout);
   const double cmu = stage_cmu(stage, dt * b.dt_scale);
-  cell_vjp_visc(b, qs, lin, cmu, c, gas);
+  cell_vjp_visc(b, qs, lin, cmu, (unsigned)c, gas);
   __builtin_amdgcn_sched_barrier(0);
-  cell_vjp_faces<LIM>(b, qs, lin, cmu, c, gas.gamma);
+  cell_vjp_faces<LIM>(b, qs, lin, cmu, (unsigned)c, gas.gamma);
 }
 
 // at most 128 VGPRs: this memory-bound gather wants four waves per SIMD
@@ -259,7 +292,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   vjp_bufs(b, stage, qs, lin, lout);
   const double c_next = stage == 3 ? 2.0 / 3.0 : 0.25;
   const double c0 = stage == 3 ? 1.0 / 3.0 : 0.75;
-  cell_vjp_gather<LIM>(b, qs, lin, lout, c, gas, c_next, c0, stage == 3 ? 0 : (stage == 2 ? 1 : 2));
+  cell_vjp_gather<LIM>(b, qs, lin, lout, (unsigned)c, gas, c_next, c0, stage == 3 ? 0 : (stage == 2 ? 1 : 2));
 }
 
 // ---------------------------------------------------------------------------

@@ -110,7 +110,8 @@ int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, do
 /* ---- plan: one multi-block configuration (Config.topology + gas + scheme) - */
 
 /* dims: nblocks x {ni, nj, nk} (padded, each >= 2*ghost; a block of 2^31 or
- * more cells is refused with FAMRDP_E_INVALID).  cuts/bcs as above.  Builds
+ * more cells, or of 2^29 or more with FAMRDP_PLAN_ADJOINT, is refused with
+ * FAMRDP_E_INVALID).  cuts/bcs as above.  Builds
  * the halo gather table (final writer per ghost cell, pre-exchange sources),
  * the per-face BC tables, the tile table and the stage workspace.
  * Replaces famrdp_jax/solver/step.py:18-56's per-call setup. */

@@ -86,31 +86,48 @@ __global__ void k_bc_face_vjp(const BlockDesc* __restrict__ blocks, const BcEntr
     for (int v = 0; v < 5; ++v) part[((int64_t)blockIdx.y * 5 + v) * maxcells + t] = acc[v];
 }
 
-// Deterministic sum of the farfield partials of one face entry per workgroup,
-// added to dqinf[(block*6 + face)*5 + v].
+// Deterministic sum of the farfield partials of each face entry, in two
+// passes: BC_RED_PARTS workgroups per entry (blockIdx.y) each sum a fixed
+// chunk of the face's cells into red[(entry * 5 + v) * BC_RED_PARTS + part],
+// then k_bc_reduce_final adds the parts of each entry in order to
+// dqinf[(block*6 + face)*5 + v].  One workgroup per face left the reduction
+// of a 10^4-cell face to 256 threads in series.
+constexpr int BC_RED_PARTS = 32;
+
 __global__ void __launch_bounds__(256) k_bc_reduce(const BlockDesc* __restrict__ blocks,
                                                    const BcEntry* __restrict__ entries, const double* __restrict__ part,
-                                                   int64_t maxcells, double* __restrict__ dqinf) {
-  const BcEntry e = entries[blockIdx.x];
+                                                   int64_t maxcells, double* __restrict__ red) {
+  const BcEntry e = entries[blockIdx.y];
   if (e.kind != BC_FARFIELD) return;
   const BlockDesc b = blocks[e.block];
   const int ax = e.face / 2;
   const int n[3] = {b.ni, b.nj, b.nk};
   const int64_t ncell = (int64_t)n[ax == 0 ? 1 : 0] * n[ax == 2 ? 1 : 2];
-  __shared__ double red[5][256];
+  const int64_t chunk = (ncell + BC_RED_PARTS - 1) / BC_RED_PARTS;
+  const int64_t t0 = blockIdx.x * chunk, t1 = t0 + chunk < ncell ? t0 + chunk : ncell;
+  __shared__ double acc[5][256];
   for (int v = 0; v < 5; ++v) {
     double s = 0.0;
-    for (int64_t t = threadIdx.x; t < ncell; t += blockDim.x) s += part[((int64_t)blockIdx.x * 5 + v) * maxcells + t];
-    red[v][threadIdx.x] = s;
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) s += part[((int64_t)blockIdx.y * 5 + v) * maxcells + t];
+    acc[v][threadIdx.x] = s;
   }
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
-      for (int v = 0; v < 5; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + s];
+      for (int v = 0; v < 5; ++v) acc[v][threadIdx.x] += acc[v][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && dqinf)
-    for (int v = 0; v < 5; ++v) dqinf[(e.block * 6 + e.face) * 5 + v] += red[v][0];
+  if (threadIdx.x < 5) red[((int64_t)blockIdx.y * 5 + threadIdx.x) * BC_RED_PARTS + blockIdx.x] = acc[threadIdx.x][0];
+}
+
+__global__ void k_bc_reduce_final(const BcEntry* __restrict__ entries, const double* __restrict__ red,
+                                  double* __restrict__ dqinf) {
+  const BcEntry e = entries[blockIdx.x];
+  const int v = threadIdx.x;
+  if (e.kind != BC_FARFIELD || v >= 5 || !dqinf) return;
+  double s = 0.0;
+  for (int q = 0; q < BC_RED_PARTS; ++q) s += red[((int64_t)blockIdx.x * 5 + v) * BC_RED_PARTS + q];
+  dqinf[(e.block * 6 + e.face) * 5 + v] += s;
 }
 
 // ---------------------------------------------------------------------------

@@ -131,6 +131,7 @@ struct famrdp_plan {
   double* d_halo_S = nullptr;
   double* d_bc_snap[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double* d_bc_part[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* d_bc_red[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // per-chunk sums of d_bc_part
   // stage timing (HIP events on the launch stream, eager launches only)
   bool timing = false;
   bool capturing = false;
@@ -546,6 +547,7 @@ int famrdp_plan_destroy(famrdp_plan* p) {
   for (int f = 0; f < 6; ++f) {
     if (p->d_bc_snap[f]) (void)hipFree(p->d_bc_snap[f]);
     if (p->d_bc_part[f]) (void)hipFree(p->d_bc_part[f]);
+    if (p->d_bc_red[f]) (void)hipFree(p->d_bc_red[f]);
   }
   for (auto* x : p->q1) (void)hipFree(x);
   for (auto* x : p->q2) (void)hipFree(x);
@@ -720,7 +722,8 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
       if (p->nbc[f] == 0) continue;
       const size_t n = (size_t)p->nbc[f] * 5 * (size_t)p->bc_maxcells[f];
       if (hipMalloc(&p->d_bc_snap[f], sizeof(double) * n * p->ghost) != hipSuccess ||
-          hipMalloc(&p->d_bc_part[f], sizeof(double) * n) != hipSuccess)
+          hipMalloc(&p->d_bc_part[f], sizeof(double) * n) != hipSuccess ||
+          hipMalloc(&p->d_bc_red[f], sizeof(double) * p->nbc[f] * 5 * BC_RED_PARTS) != hipSuccess)
         return cleanup(fail(FAMRDP_E_HIP, "plan_create: adjoint BC buffers failed"));
     }
   }
@@ -901,7 +904,11 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
     k_bc_face_vjp<<<grid, 256, 0, st>>>(p->d_blocks_adj, p->d_bc[f], p->ghost, p->gas.gamma, p->gas.R,
                                         p->d_bc_snap[f], p->d_bc_part[f], p->bc_maxcells[f]);
     LAUNCH_CHECK();
-    k_bc_reduce<<<p->nbc[f], 256, 0, st>>>(p->d_blocks_adj, p->d_bc[f], p->d_bc_part[f], p->bc_maxcells[f], dqinf);
+    k_bc_reduce<<<dim3(BC_RED_PARTS, (unsigned)p->nbc[f]), 256, 0, st>>>(p->d_blocks_adj, p->d_bc[f],
+                                                                        p->d_bc_part[f], p->bc_maxcells[f],
+                                                                        p->d_bc_red[f]);
+    LAUNCH_CHECK();
+    k_bc_reduce_final<<<p->nbc[f], 64, 0, st>>>(p->d_bc[f], p->d_bc_red[f], dqinf);
     LAUNCH_CHECK();
   }
   if (p->nhalo > 0) {

@@ -873,26 +873,43 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
   // and q_start is read by the copy below
   int rc = bind(p, nullptr, kxyz, vol, st, false);
   if (rc) return rc;
+  // The adjoint table reads the caller's arrays where it can instead of
+  // copies of them: the stage states when supplied (read only), lam as the
+  // cotangent entering stage 3 (stage 2 overwrites it with its own, and the
+  // result is written to it at the end), q_start itself as the state the
+  // halo and BCs are applied to when the caller gives it up.
+  const bool consume = (flags & FAMRDP_VJP_CONSUME_START) != 0;
+  bool rebind = false;
+  for (int b = 0; b < p->nblocks; ++b) {
+    BlockDesc& a = p->h_blocks_adj[b];
+    double* qh = consume ? q_start[b] : p->adj_bufs[6 * b];   // adj_bufs: qh, la, lb, l0, fs, vs per block
+    double* q1 = q1s ? const_cast<double*>(q1s[b]) : p->h_blocks[b].q1;
+    double* q2 = q1s ? const_cast<double*>(q2s[b]) : p->h_blocks[b].q2;
+    if (a.qh != qh || a.q != qh || a.q1 != q1 || a.q2 != q2 || a.la != lam[b]) rebind = true;
+    a.qh = qh;
+    a.q = qh;
+    a.q1 = q1;
+    a.q2 = q2;
+    a.la = lam[b];
+  }
+  if (rebind) {
+    rc = upload_table(p, p->d_blocks_adj, p->h_blocks_adj, st);
+    if (rc) return rc;
+  }
   // ---- q0' = BC(halo(q_start)) into qh (BC snapshots kept), then stages 1-2
-  rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].qh; }, [&](int b) { return q_start[b]; });
-  if (rc) return rc;
+  if (!consume) {
+    rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].qh; }, [&](int b) { return q_start[b]; });
+    if (rc) return rc;
+  }
   rc = launch_halo(p, st, p->d_blocks_adj);
   if (rc) return rc;
   rc = launch_bc(p, st, p->d_blocks_adj, true);
   if (rc) return rc;
-  if (q1s) {
-    rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].q1; }, [&](int b) { return q1s[b]; });
-    if (rc) return rc;
-    rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].q2; }, [&](int b) { return q2s[b]; });
-    if (rc) return rc;
-  } else {
-    rc = launch_stages(p, dt, flags, st, true, 2);
+  if (!q1s) {
+    rc = launch_stages(p, dt, flags & ~FAMRDP_VJP_CONSUME_START, st, true, 2);
     if (rc) return rc;
   }
   // ---- reverse the three stages
-  rc = copy_blocks(p, st, [&](int b) { return p->h_blocks_adj[b].la; },
-                   [&](int b) { return static_cast<const double*>(lam[b]); });
-  if (rc) return rc;
   for (int stage = 3; stage >= 1; --stage) {
     launch_stage_vjp(p->d_blocks_adj, p->d_tiles, p->ntiles, stage, p->gas, dt, p->limiter, st);
     LAUNCH_CHECK();

@@ -148,11 +148,14 @@ class Plan:
         L.check(self._lib.famrdp_plan_timing(self._h, 0, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
-    def step_vjp(self, q_start, kxyzs, vols, lams, dt, dqinf=None, *, batched=False, direct=False, stages=None):
+    def step_vjp(self, q_start, kxyzs, vols, lams, dt, dqinf=None, *, batched=False, direct=False, stages=None,
+                 consume_start=False):
         """In place: lams (cotangent of the step's result) -> cotangent of
         q_start.  ``stages`` = (q1s, q2s) saved by stage_states after the
-        forward step from q_start; without it they are recomputed."""
-        flags = (1 if batched else 0) | (2 if direct else 0)
+        forward step from q_start; without it they are recomputed.
+        consume_start: q_start may be overwritten (its ghost cells receive the
+        halo and BCs) instead of being copied first."""
+        flags = (1 if batched else 0) | (2 if direct else 0) | (4 if consume_start else 0)
         if stages is None:
             L.check(self._lib.famrdp_step_vjp(self._h, self._ptrs(q_start), self._ptrs(kxyzs), self._ptrs(vols),
                                               self._ptrs(lams), float(dt), L.ptr(dqinf), flags, L.stream()))

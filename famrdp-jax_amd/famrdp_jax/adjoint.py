@@ -128,7 +128,8 @@ class Adjoint:
                         stages.append((q1s, q2s))
             for n in reversed(range(seg_end - seg_start)):
                 self.plan.step_vjp(starts[n], self.kxyz, self.vol, lam, dt, dqinf, batched=self.batched,
-                                   direct=self.direct, stages=stages[n] if keep_stages else None)
+                                   direct=self.direct, stages=stages[n] if keep_stages else None,
+                                   consume_start=True)
                 starts[n] = None
                 if keep_stages:
                     stages[n] = None    # release as the reverse pass goes
@@ -157,7 +158,7 @@ class Adjoint:
             t1 = time.perf_counter()
             for n in reversed(range(nsteps)):
                 self.plan.step_vjp(starts[n], self.kxyz, self.vol, lam, dt, dqinf, batched=self.batched,
-                                   direct=self.direct, stages=stages[n])
+                                   direct=self.direct, stages=stages[n], consume_start=True)
                 starts[n] = stages[n] = None    # release as the reverse pass goes
         else:
             cur, checkpoints, last = self.forward_checkpointed(self.q0, dt, nsteps, checkpoint_every,

@@ -125,6 +125,8 @@ int famrdp_plan_destroy(famrdp_plan* plan);
 /* Step variants (flags of famrdp_step). */
 #define FAMRDP_STEP_BATCHED 1   /* one launch per kernel across all blocks (block-offset/tile table) */
 #define FAMRDP_STEP_DIRECT 2    /* reference-order direct kernels (one thread per cell, no LDS); for cross-checks */
+#define FAMRDP_VJP_CONSUME_START 4  /* famrdp_step_vjp*: q_start may be overwritten (it receives the halo and BCs
+                                       the reverse sweep linearises at) instead of being copied first */
 
 /* famrdp_jax/solver/step.py:18-56: halo (mesh/halo.py:56), BCs
  * (physics/bc/dispatch.py:11), then Shu-Osher RK3 on every padded cell,

@@ -30,10 +30,14 @@ class _Plan:
             a.copy_(q + 0.25)
             b.copy_(q + 0.5)
 
-    def step_vjp(self, starts, kxyz, vol, lam, dt, dqinf, batched=False, direct=False, stages=None):
+    def step_vjp(self, starts, kxyz, vol, lam, dt, dqinf, batched=False, direct=False, stages=None,
+                 consume_start=False):
         s = float(starts[0].flatten()[0])
         tags = None if stages is None else (float(stages[0][0].flatten()[0]), float(stages[1][0].flatten()[0]))
         self.vjps.append((s, tags))
+        if consume_start:   # the plan may overwrite a consumed start: the driver must not read it again
+            for q in starts:
+                q += 1000.0
 
 
 def _adjoint():

@@ -221,6 +221,44 @@ def test_kept_stage_states_equal_recomputed():
     np.testing.assert_array_equal(dq2.reshape(-1).cpu().numpy(), outs[0][1])
 
 
+@pytest.mark.parametrize("batched", [False, True])
+def test_step_vjp_consumed_or_copied_start_same_bits(batched):
+    """One famrdp_step_vjp four ways -- q_start copied or consumed
+    (FAMRDP_VJP_CONSUME_START), stage states recomputed or supplied -- gives
+    the same cotangent and q_inf cotangent bit for bit; without the flag
+    q_start is left as it was, with it only its ghost cells change (the halo
+    and BCs the reverse sweep linearises at)."""
+    print(f"case: C4-like 12^3, one reverse step, batched={batched}, copied/consumed start", flush=True)
+    state, cfg, dt = _c4_small(12)
+    adj = Adjoint(to_dev(state), cfg, batched=batched)
+    start = [q.clone() for q in adj.q0]
+    cur = [q.clone() for q in adj.q0]
+    adj._step(cur, dt)
+    q1s = [torch.empty_like(q) for q in cur]
+    q2s = [torch.empty_like(q) for q in cur]
+    adj.plan.stage_states(q1s, q2s)
+    _, lam0 = adj.objective(cur, OBJ_WALL_PRESSURE)
+    outs = []
+    for consume in (False, True):
+        for stages in (None, (q1s, q2s)):
+            qs = [q.clone() for q in start]
+            lam = [l.clone() for l in lam0]
+            dq = torch.zeros(6 * 5, dtype=torch.float64, device="cuda")
+            adj.plan.step_vjp(qs, adj.kxyz, adj.vol, lam, dt, dq, batched=batched, stages=stages,
+                              consume_start=consume)
+            torch.cuda.synchronize()
+            if not consume:
+                assert torch.equal(qs[0], start[0])
+            else:
+                g = cfg.ghost
+                assert torch.equal(qs[0][:, g:-g, g:-g, g:-g], start[0][:, g:-g, g:-g, g:-g])
+            outs.append((lam[0].cpu().numpy(), dq.cpu().numpy()))
+    for lam_o, dq_o in outs[1:]:
+        np.testing.assert_array_equal(lam_o, outs[0][0])
+        np.testing.assert_array_equal(dq_o, outs[0][1])
+    assert np.isfinite(outs[0][0]).all() and np.abs(outs[0][0]).max() > 0
+
+
 def test_torch_autograd_function():
     print("case: torch.autograd.Function wrapper", flush=True)
     st, cfg = cases.case_c1(n=8, seed=9, p_noise=1e-3)

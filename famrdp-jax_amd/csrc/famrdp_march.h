@@ -26,6 +26,11 @@
 //      far-end layer, dealt over all threads as one list) go to the face
 //      buffer; the halo of plane s+1 is converted for the next step.
 //
+// Loads are issued a phase or more before their use where registers allow:
+// the column's metrics of plane s before the barrier of phase a (and kept for
+// the next step's divergence), the update operands of plane s-1 and the
+// cross halo's metrics in phase b, the state of plane s+2 a step ahead.
+//
 // Each cell's primitives, viscous tensor and faces are computed once per
 // segment, apart from the tile's halo rings in j and k and the two extra
 // planes at each segment start.  Cell coordinates are fixed per thread for
@@ -336,7 +341,10 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
       rhs[2] = inv[2] + vis[1];
       rhs[3] = inv[3] + vis[2];
       rhs[4] = inv[4] + vis[3];
-      march_keep(qs);    // loaded in phase b: combined here as values, as loads
+      // loaded in phase b; held opaque here so that the combination below
+      // compiles with the same contractions as the fused kernel's, which
+      // loads these values right where it combines them
+      march_keep(qs);
       march_keep(qq0);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {

@@ -46,16 +46,19 @@ def test_variant_count_matches_header():
 
 def test_plan_create_rejects_bad_block_sizes_before_allocating():
     """famrdp_plan_create validates every block before any HIP call: padded
-    sizes below 2*ghost and blocks of 2^31 or more cells (the tile kernels
-    index cells in 32 bits) return FAMRDP_E_INVALID with a message."""
+    sizes below 2*ghost, blocks of 2^31 or more cells (the tile kernels
+    index cells in 32 bits) and, in a plan with FAMRDP_PLAN_ADJOINT (2),
+    blocks of 2^29 or more (the reverse kernels' 32-bit byte offsets) return
+    FAMRDP_E_INVALID with a message."""
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("library not built (run __graft_entry__.build())")
     L = _lib.load()
     gas = _lib.Gas(1.4, 1.0 / 1.4, 1e-3, 1.0, 0.38, 0.72)
-    for dims, msg in (((3, 10, 10), b"2*ghost"), ((2048, 1100, 1000), b"2^31")):
+    for dims, flags, msg in (((3, 10, 10), 0, b"2*ghost"), ((2048, 1100, 1000), 0, b"2^31"),
+                             ((1024, 1024, 512), 2, b"2^29 or more cells in an adjoint plan")):
         plan = _lib.vp()
         d = (_lib.i32 * 3)(*dims)
-        rc = L.famrdp_plan_create(ctypes.byref(plan), 1, d, 0, None, 0, None, ctypes.byref(gas), 0, 2, 0)
+        rc = L.famrdp_plan_create(ctypes.byref(plan), 1, d, 0, None, 0, None, ctypes.byref(gas), 0, 2, flags)
         assert rc == _lib.E_INVALID and plan.value is None
         assert msg in L.famrdp_last_error()
 

@@ -40,8 +40,8 @@
 // roll of the viscous terms reads; only columns inside the block write.  A
 // narrow k remainder goes to transposed TK x TJ strip tiles (march_tiling),
 // so that it does not take a row of TJ x TK tiles owning a few columns each.
-// Loads address the arrays with 32-bit byte offsets (uniform base, one
-// vector offset), so a block's kxyz must stay under 4 GiB (march_fits).
+// Loads and stores go through buffer resources with 32-bit byte offsets, so
+// a block's kxyz must stay under 4 GiB (march_fits).
 #pragma once
 
 #include "famrdp_march_geom.h"
@@ -55,18 +55,32 @@ DF void march_prim(const double q[5], double g, double R, double pv[6]) {
   pv[5] = temperature_fast(pv, R);
 }
 
-// a[i] through a 32-bit byte offset from a uniform base (i < 2^29), as a
-// global load
-DF double ldx(const double* __restrict__ a, unsigned i) {
-  return *(gmem_cdouble*)(reinterpret_cast<const char*>(a) + (i << 3));
+// The arrays a march reads and writes, as buffer resources: a load or store
+// takes the element's 32-bit byte offset in a vector register and the
+// variable's offset v N (bytes) in a scalar register, so no address is formed
+// per access (global loads from base + v N formed a 64-bit vector address for
+// every variable but the first: 1 % of the stage's time).  Every array of a
+// block that march_fits accepts spans fewer than 2^32 bytes, which is the
+// resource's record count; the last word is the gfx9-family setting for
+// untyped (raw) access (32-bit data format, stride 0).
+typedef unsigned march_u2 __attribute__((__vector_size__(8)));
+
+DF __amdgpu_buffer_rsrc_t march_rsrc(const double* p, int64_t n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), 0, (int)(unsigned)(n * 8), 0x00020000);
 }
 
-// a[i] = x, the same way
-DF void stx(double* a, unsigned i, double x) { *(gmem_double*)(reinterpret_cast<char*>(a) + (i << 3)) = x; }
+// element i (< 2^29) of variable v: byte offset i * 8 + soff, soff = v N * 8
+DF double ldx(__amdgpu_buffer_rsrc_t r, unsigned i, unsigned soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, i << 3, (int)soff, 0));
+}
 
-DF void march_load5(const double* __restrict__ a, int64_t N, unsigned off, double x[5]) {
+DF void stx(__amdgpu_buffer_rsrc_t r, unsigned i, unsigned soff, double x) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(march_u2, x), r, i << 3, (int)soff, 0);
+}
+
+DF void march_load5(__amdgpu_buffer_rsrc_t r, unsigned nb, unsigned off, double x[5]) {
 #pragma unroll
-  for (int v = 0; v < 5; ++v) x[v] = ldx(a + v * N, off);
+  for (int v = 0; v < 5; ++v) x[v] = ldx(r, off, v * nb);
 }
 
 // kxyz of the largest block a marching launch may address (9 N doubles
@@ -104,8 +118,8 @@ DF void march_visc(const double* sP, int sl, const double um[4], const double up
 // c - 1 and c of row r.  Faces exist between cells 1 .. n-2 of their axis
 // (left cell a in [1, n-3]) and never read a wrapped cell.
 template <int TJ, int TK, int LIM>
-DF void march_faces(const BlockDesc& b, int j0, int k0, unsigned op, const double* sP, double* sF, double g,
-                    int t) {
+DF void march_faces(const BlockDesc& b, __amdgpu_buffer_rsrc_t rk, unsigned nb, int j0, int k0, unsigned op,
+                    const double* sP, double* sF, double g, int t) {
   using M = MarchGeom<TJ, TK>;
   const int nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
@@ -128,8 +142,8 @@ DF void march_faces(const BlockDesc& b, int j0, int k0, unsigned op, const doubl
     double kl[3], kr[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      kl[m] = ldx(b.kxyz + m * N, cl);
-      kr[m] = ldx(b.kxyz + m * N, cr);
+      kl[m] = ldx(rk, cl, m * nb);
+      kr[m] = ldx(rk, cr, m * nb);
     }
     double Fa[5];
     face_flux_prim<LIM>(p4[0], p4[1], p4[2], p4[3], kl, kr, g, Fa);
@@ -176,7 +190,10 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
   Gas gb = gas;
   gb.mu0 = b.mu0;
   dt = dt * b.dt_scale;
-  const double* __restrict__ kx = b.kxyz;
+  const unsigned nb = (unsigned)N * 8u;     // bytes between variables
+  const __amdgpu_buffer_rsrc_t rq = march_rsrc(qin, 5 * N), r0 = march_rsrc(q0, 5 * N),
+                               ro = march_rsrc(qout, 5 * N), rk = march_rsrc(b.kxyz, 9 * N),
+                               rv = march_rsrc(b.vol, N);
 
   // ---- fixed per thread for the whole march --------------------------------
   const MarchThread th = M::thread(t, j0, k0, nj, nk);
@@ -190,14 +207,14 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
   double pm[6], pc[6];
   {
     double qq[5], x[6];
-    march_load5(qin, N, plane(pA - 2) + col, qq);
+    march_load5(rq, nb, plane(pA - 2) + col, qq);
     march_prim(qq, g, gb.R, pm);
-    march_load5(qin, N, plane(pA - 1) + col, qq);
+    march_load5(rq, nb, plane(pA - 1) + col, qq);
     march_prim(qq, g, gb.R, pc);
     march_keep(pm);
     march_keep(pc);
     if (hg) {
-      march_load5(qin, N, plane(pA - 2) + hcol, qq);
+      march_load5(rq, nb, plane(pA - 2) + hcol, qq);
       march_prim(qq, g, gb.R, x);
       sU[hs] = x[1];
       sU[M::NHG + hs] = x[2];
@@ -205,15 +222,15 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
       sU[3 * M::NHG + hs] = x[5];
     }
     if (hs >= 0) {
-      march_load5(qin, N, plane(pA - 1) + hcol, qq);
+      march_load5(rq, nb, plane(pA - 1) + hcol, qq);
       march_prim(qq, g, gb.R, x);
 #pragma unroll
       for (int v = 0; v < 6; ++v) sH[v * M::NHS + hs] = x[v];
     }
   }
   double qn[5], qnh[5] = {0, 0, 0, 0, 0};
-  march_load5(qin, N, plane(pA) + col, qn);
-  if (hs >= 0) march_load5(qin, N, plane(pA) + hcol, qnh);
+  march_load5(rq, nb, plane(pA) + col, qn);
+  if (hs >= 0) march_load5(rq, nb, plane(pA) + hcol, qnh);
 
   double kpv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // kxyz of the column at plane s-1
   double pLn[5] = {0, 0, 0, 0, 0};       // left state of the column's next i-face
@@ -231,7 +248,7 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
     // divergence of G(s)
     double kk[9];
 #pragma unroll
-    for (int m = 0; m < 9; ++m) kk[m] = ldx(kx + m * N, c_s);
+    for (int m = 0; m < 9; ++m) kk[m] = ldx(rk, c_s, m * nb);
     // ---- a. P <- plane s ------------------------------------------------------
 #pragma unroll
     for (int v = 0; v < 6; ++v) sP[v * M::NP + slot] = pc[v];
@@ -245,7 +262,7 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
     double pn[6];
     march_prim(qn, g, gb.R, pn);
     march_keep(pn);
-    if (s + 2 <= pB + 1) march_load5(qin, N, plane(s + 2) + march_opaque(col), qn);
+    if (s + 2 <= pB + 1) march_load5(rq, nb, plane(s + 2) + march_opaque(col), qn);
     double Fi[5] = {0, 0, 0, 0, 0};
     {
       double kl[3], kr[3];
@@ -266,11 +283,11 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
     // the update operands of plane s-1, issued before G(s) is computed
     double qs[5] = {0, 0, 0, 0, 0}, qq0[5] = {0, 0, 0, 0, 0}, vl = 1.0;
     if (upd && own) {
-      vl = ldx(b.vol, c_m);
+      vl = ldx(rv, c_m, 0);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        qs[v] = ldx(qin + v * N, c_m);
-        qq0[v] = stage == 1 ? 0.0 : ldx(q0 + v * N, c_m);
+        qs[v] = ldx(rq, c_m, v * nb);
+        qq0[v] = stage == 1 ? 0.0 : ldx(r0, c_m, v * nb);
       }
     }
     double g9[9];
@@ -284,7 +301,7 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
     if (hg) {
       const unsigned ch = op + march_opaque(hcol);
 #pragma unroll
-      for (int m = 0; m < 9; ++m) kh[m] = ldx(kx + m * N, ch);
+      for (int m = 0; m < 9; ++m) kh[m] = ldx(rk, ch, m * nb);
     }
 #pragma unroll
     for (int v = 0; v < 6; ++v) {
@@ -348,7 +365,7 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
       march_keep(qq0);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        stx(qout + v * N, c_m, rk_combine(stage, qq0[v], qs[v], dt, rhs[v]));
+        stx(ro, c_m, v * nb, rk_combine(stage, qq0[v], qs[v], dt, rhs[v]));
       }
     }
 #pragma unroll
@@ -362,7 +379,7 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
       double x[6];
       march_prim(qnh, g, gb.R, x);     // plane s+1 at the halo cell
       march_keep(x);
-      if (s + 2 <= pB + 1) march_load5(qin, N, plane(s + 2) + march_opaque(hcol), qnh);
+      if (s + 2 <= pB + 1) march_load5(rq, nb, plane(s + 2) + march_opaque(hcol), qnh);
 #pragma unroll
       for (int v = 0; v < 6; ++v) sH[v * M::NHS + hs] = x[v];
       if (hg) {
@@ -378,7 +395,7 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
         sU[3 * M::NHG + hs] = sP[5 * M::NP + hslot];
       }
     }
-    if (faces) march_faces<TJ, TK, LIM>(b, j0, k0, op, sP, sF, g, (int)march_opaque((unsigned)t));
+    if (faces) march_faces<TJ, TK, LIM>(b, rk, nb, j0, k0, op, sP, sF, g, (int)march_opaque((unsigned)t));
     __syncthreads();   // G(s) and the faces of plane s in place; P free
   }
 }

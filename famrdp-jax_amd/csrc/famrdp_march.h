@@ -22,9 +22,10 @@
 //      j and k from the LDS face buffer filled in the previous step) and the
 //      Shu-Osher combination; plane s-1 is written.
 //   e. G(s) replaces G(s-1) in LDS, with G of the tile's one-cell cross halo;
-//      the MUSCL-2 + Roe faces of plane s along j and k (the tile's and the
-//      far-end layer, dealt over all threads as one list) go to the face
-//      buffer; the halo of plane s+1 is converted for the next step.
+//      the MUSCL-2 + Roe faces of plane s along j and k go to the face
+//      buffer (each thread its column's low j-face, then its low k-face,
+//      both with offsets fixed by the column; the last wave the far-end
+//      layer); the halo of plane s+1 is converted for the next step.
 //
 // Loads are issued a phase or more before their use where registers allow:
 // the column's metrics of plane s before the barrier of phase a (and kept for
@@ -112,45 +113,51 @@ DF void march_visc(const double* sP, int sl, const double um[4], const double up
   viscous_flux_fast(pv, Tc, d, kc, gb, g9);
 }
 
-// j and k faces of the plane at offset `op` (rhs/invscd.py:53-97) into the
-// face buffer: j-face f (< NFJ) at (f / TK, f % TK) lies between local rows
-// f / TK - 1 and f / TK; k-face NFJ + r (TK + 1) + c between local columns
-// c - 1 and c of row r.  Faces exist between cells 1 .. n-2 of their axis
-// (left cell a in [1, n-3]) and never read a wrapped cell.
+// One j- or k-face of the plane (rhs/invscd.py:53-97) into face buffer slot
+// f: the four cells' primitives from P slots base + s str (s = 0..3), kxyz
+// of the face's axis at the cells either side (plane offsets cl, cr; the
+// axis's first row of kxyz at byte offset s0).
 template <int TJ, int TK, int LIM>
-DF void march_faces(const BlockDesc& b, __amdgpu_buffer_rsrc_t rk, unsigned nb, int j0, int k0, unsigned op,
-                    const double* sP, double* sF, double g, int t) {
+DF void march_face(const double* sP, double* sF, int base, int str, __amdgpu_buffer_rsrc_t rk, unsigned cl,
+                   unsigned cr, unsigned s0, unsigned nb, int f, double g) {
   using M = MarchGeom<TJ, TK>;
-  const int nj = b.nj, nk = b.nk;
-  const int64_t N = b.N;
+  double p4[4][5];
 #pragma unroll
-  for (int r = 0; r < M::NFR; ++r) {
-    const int f = M::face_of(t, r);
-    if (f < 0) continue;
-    const MarchFace fc = M::face(f, j0, k0, nj, nk);
-    if (!fc.ok) continue;
-    const bool isj = fc.isj;
-    const int a = fc.a, oc = fc.oc, base = fc.base, str = fc.str;
-    double p4[4][5];
+  for (int s4 = 0; s4 < 4; ++s4) {
 #pragma unroll
-    for (int s4 = 0; s4 < 4; ++s4) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) p4[s4][v] = sP[v * M::NP + base + s4 * str];
-    }
-    const unsigned cl = op + (isj ? (unsigned)a * nk + oc : (unsigned)oc * nk + a) + (isj ? 3u : 6u) * (unsigned)N;
-    const unsigned cr = cl + (isj ? (unsigned)nk : 1u);
-    double kl[3], kr[3];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      kl[m] = ldx(rk, cl, m * nb);
-      kr[m] = ldx(rk, cr, m * nb);
-    }
-    double Fa[5];
-    face_flux_prim<LIM>(p4[0], p4[1], p4[2], p4[3], kl, kr, g, Fa);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) sF[v * M::NFB + f] = Fa[v];
-    __builtin_amdgcn_sched_barrier(0);   // one face at a time: the rounds are not interleaved
+    for (int v = 0; v < 5; ++v) p4[s4][v] = sP[v * M::NP + base + s4 * str];
   }
+  double kl[3], kr[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    kl[m] = ldx(rk, cl, s0 + m * nb);
+    kr[m] = ldx(rk, cr, s0 + m * nb);
+  }
+  double Fa[5];
+  face_flux_prim<LIM>(p4[0], p4[1], p4[2], p4[3], kl, kr, g, Fa);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sF[v * M::NFB + f] = Fa[v];
+}
+
+// The far-end layer of the plane at offset `op` (face_of round 2: j-faces
+// past the tile's last row, k-faces past its last column), for the threads
+// of the last wave; j-face f (< NFJ) at (f / TK, f % TK) lies between local
+// rows f / TK - 1 and f / TK, k-face NFJ + r (TK + 1) + c between local
+// columns c - 1 and c of row r.  Faces exist between cells 1 .. n-2 of their
+// axis (left cell a in [1, n-3]) and never read a wrapped cell.
+template <int TJ, int TK, int LIM>
+DF void march_far_faces(const BlockDesc& b, __amdgpu_buffer_rsrc_t rk, unsigned nb, int j0, int k0, unsigned op,
+                        const double* sP, double* sF, double g, int t) {
+  using M = MarchGeom<TJ, TK>;
+  const int f = M::face_of(t, 2);
+  if (f < 0) return;
+  const int nj = b.nj, nk = b.nk;
+  const MarchFace fc = M::face(f, j0, k0, nj, nk);
+  if (!fc.ok) return;
+  const bool isj = fc.isj;
+  const unsigned cl = op + (isj ? (unsigned)fc.a * nk + fc.oc : (unsigned)fc.oc * nk + fc.a) + (isj ? 3u : 6u) * (unsigned)b.N;
+  const unsigned cr = cl + (isj ? (unsigned)nk : 1u);
+  march_face<TJ, TK, LIM>(sP, sF, fc.base, fc.str, rk, cl, cr, 0, nb, f, g);
 }
 
 // v, hidden from the optimiser: what is computed from it inside the march
@@ -199,7 +206,7 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
   const MarchThread th = M::thread(t, j0, k0, nj, nk);
   const int tj = th.tj, tk = th.tk, slot = th.slot, gslot = th.gslot, hs = th.hs;
   const int hslot = th.hslot, hgslot = th.hgslot;
-  const bool own = th.own, jin = th.jin, kin = th.kin, hg = th.hg;
+  const bool own = th.own, jin = th.jin, kin = th.kin, fj = th.fj, fk = th.fk, hg = th.hg;
   const unsigned col = th.col, hcol = th.hcol;
   auto plane = [&](int s) -> unsigned { return (unsigned)(s < 0 ? s + ni : (s >= ni ? s - ni : s)) * si; };
 
@@ -395,7 +402,18 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
         sU[3 * M::NHG + hs] = sP[5 * M::NP + hslot];
       }
     }
-    if (faces) march_faces<TJ, TK, LIM>(b, rk, nb, j0, k0, op, sP, sF, g, (int)march_opaque((unsigned)t));
+    if (faces) {
+      // round 0: the column's low j-face (cells gj-2 .. gj+1 of column gk;
+      // in the block wherever the face exists, so its plane offsets are
+      // those of the column and the column nk before it)
+      if (fj) march_face<TJ, TK, LIM>(sP, sF, slot - 2 * M::PK, M::PK, rk, c_s - (unsigned)nk, c_s, 3 * nb, nb, t, g);
+      __builtin_amdgcn_sched_barrier(0);   // one face at a time: the rounds are not interleaved
+      // round 1: the column's low k-face
+      if (fk)
+        march_face<TJ, TK, LIM>(sP, sF, slot - 2, 1, rk, c_s - 1u, c_s, 6 * nb, nb, M::NFJ + tj * (TK + 1) + tk, g);
+      __builtin_amdgcn_sched_barrier(0);
+      march_far_faces<TJ, TK, LIM>(b, rk, nb, j0, k0, op, sP, sF, g, (int)march_opaque((unsigned)t));
+    }
     __syncthreads();   // G(s) and the faces of plane s in place; P free
   }
 }

@@ -21,11 +21,13 @@ struct MarchFace {
 // column (tj, tk) at block coordinates (gj, gk) (unwrapped; own = inside the
 // block), the column's offset in a plane at its periodic image, its P and G
 // slots, whether it takes the j / k inviscid terms (rhs/invscd.py:129-131);
-// the halo cell hs of P it stages (-1: none) with its plane offset and P
-// slot, and, for hs < NHG (hg), the G cross-halo cell of the same column.
+// whether its column's low j-face and low k-face exist (fj, fk: the faces of
+// rounds 0 and 1, see face_of); the halo cell hs of P it stages (-1: none)
+// with its plane offset and P slot, and, for hs < NHG (hg), the G cross-halo
+// cell of the same column.
 struct MarchThread {
   int tj, tk, gj, gk, slot, gslot, hs, hslot, hgslot;
-  bool own, jin, kin, hg;
+  bool own, jin, kin, fj, fk, hg;
   unsigned col, hcol;
 };
 
@@ -102,14 +104,16 @@ struct MarchGeom {
   static constexpr int NLDS = 6 * NP + 9 * NGP + 5 * NFB + 6 * NHS + 4 * NHG;
   // Thread roles.  Threads [0, NHG) also stage and compute G at the cross
   // halo cell of the same index; the other halo cells of P are staged by the
-  // threads right after them (from T1 = NHG); faces go out in rounds of NT,
-  // the last (partial) round to the threads from TF on -- the last wave(s),
-  // which then stage no halo cell when the halo fits below them.
+  // threads right after them (from T1 = NHG).  Faces go out in three rounds:
+  // round 0 the low j-face of each thread's column, round 1 its low k-face,
+  // round 2 the far-end layer (the j-faces past the last row, the k-faces
+  // past the last column: NFL = TJ + TK faces) on the last wave (threads from
+  // TF), which stages no halo cell.
   static constexpr int T1 = NHG;
-  static constexpr int NFR = (NFB + NT - 1) / NT;
-  static constexpr int NFL = NFB - (NFR - 1) * NT;
-  static constexpr int TF = NT - ((NFL + 63) / 64) * 64;
-  static_assert(NT % 64 == 0 && T1 + (NHS - NHG) <= NT && TF >= 0, "tile shape");
+  static constexpr int NFR = 3;
+  static constexpr int NFL = TJ + TK;
+  static constexpr int TF = NT - 64;
+  static_assert(NT % 64 == 0 && NFL <= 64 && T1 + (NHS - NHG) <= TF && NFB == 2 * NT + NFL, "tile shape");
 
   __host__ __device__ static constexpr int ps(int lj, int lk) { return (lj + 2) * PK + lk + 2; }
   __host__ __device__ static constexpr int gs(int lj, int lk) { return (lj + 1) * GK + lk + 1; }
@@ -140,8 +144,11 @@ struct MarchGeom {
   }
   // face of round r handled by thread t, or -1
   __host__ __device__ static constexpr int face_of(int t, int r) {
-    if (r < NFR - 1) return t + r * NT;
-    return (t >= TF && t - TF < NFL) ? (NFR - 1) * NT + (t - TF) : -1;
+    if (r == 0) return t;                                     // j-face below row tj at column tk
+    if (r == 1) return NFJ + (t / TK) * (TK + 1) + t % TK;    // k-face left of column tk in row tj
+    const int u = t - TF;
+    if (u < 0 || u >= NFL) return -1;
+    return u < TK ? TJ * TK + u : NFJ + (u - TK) * (TK + 1) + TK;
   }
   __host__ __device__ static inline MarchThread thread(int t, int j0, int k0, int nj, int nk) {
     MarchThread m{};
@@ -155,6 +162,8 @@ struct MarchGeom {
     m.gslot = gs(m.tj, m.tk);
     m.jin = m.own && m.gj >= 2 && m.gj <= nj - 3;
     m.kin = m.own && m.gk >= 2 && m.gk <= nk - 3;
+    m.fj = m.gk < nk && m.gj >= 2 && m.gj <= nj - 2;
+    m.fk = m.gj < nj && m.gk >= 2 && m.gk <= nk - 2;
     m.hs = halo_of(t);
     m.hg = t < NHG;
     m.hcol = 0;

@@ -115,6 +115,20 @@ void check_tile(int ni, int nj, int nk, int L, int j0, int k0, Acc& acc, std::ve
           const int f = M::face_of(t, r);
           if (f < 0) continue;
           const MarchFace fc = M::face(f, j0, k0, nj, nk);
+          // rounds 0 and 1: the column's own low j- and k-face, which the
+          // kernel addresses from the column's slot and plane offset
+          if (r < 2) {
+            const MarchThread th = M::thread(t, j0, k0, nj, nk);
+            acc.ok(fc.isj == (r == 0) && fc.ok == (r == 0 ? th.fj : th.fk));
+            acc.ok(f == (r == 0 ? t : M::NFJ + th.tj * (TK + 1) + th.tk));
+            if (fc.ok) {
+              acc.ok(fc.base == th.slot - (r == 0 ? 2 * M::PK : 2) && fc.str == (r == 0 ? M::PK : 1));
+              const int64_t cl = r == 0 ? (int64_t)fc.a * nk + fc.oc : (int64_t)fc.oc * nk + fc.a;
+              acc.ok(cl == (int64_t)th.col - (r == 0 ? nk : 1));
+            }
+          } else {
+            acc.ok(t >= M::TF && M::halo_of(t) < 0);
+          }
           if (!fc.ok) continue;
           for (int s4 = 0; s4 < 4; ++s4) {
             const int c = fc.a - 1 + s4;          // unwrapped along the face axis

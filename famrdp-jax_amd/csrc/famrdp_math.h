@@ -73,14 +73,15 @@ FD double limit(int lim, double a, double b) {
 // minmod without the sign products: same-sign arguments give the smaller
 // magnitude with a's sign, anything else gives zero (the reference's
 // s * min(|a|,|b|) with s in {-1, 0, 1}; a zero's sign never reaches q).
-// As max(min(a,b),0) + min(max(a,b),0): both positive gives min(a,b) + 0,
-// both negative 0 + max(a,b) (the one nearer zero), mixed signs or a zero
-// give 0 -- the reference's value exactly, in four min/max and one add, for
-// non-NaN arguments.  A NaN argument is dropped here (hardware min/max); the
-// face routine that calls it (face_flux_prim) restores the reference's NaN
-// propagation for the whole four-cell stencil instead, at two instructions
-// per face rather than two per limiter call.
-FD double minmod_fast(double a, double b) { return fmax(fmin(a, b), 0.0) + fmin(fmax(a, b), 0.0); }
+// As the median of (a, b, 0), max(min(a,b), min(max(a,b),0)): both positive
+// gives max(min(a,b), 0) = min(a,b), both negative max(min(a,b), max(a,b)) =
+// max(a,b) (the one nearer zero), mixed signs or a zero give 0 -- the
+// reference's value exactly, in four min/max, for non-NaN arguments.  A NaN
+// argument is dropped here (hardware min/max return the other operand; two
+// NaNs give 0); the face routine that calls it (face_flux_prim) restores the
+// reference's NaN propagation for the whole four-cell stencil instead, at two
+// instructions per face rather than two per limiter call.
+FD double minmod_fast(double a, double b) { return fmax(fmin(a, b), fmin(fmax(a, b), 0.0)); }
 
 // flux/roe.py:5-18
 FD void physical_flux(const double U[5], const double n[3], double g, double F[5]) {
@@ -351,10 +352,10 @@ DF void face_state_fence(double x[5]) {
 }
 
 // Area-weighted Roe flux of one face from its reconstructed left/right
-// primitives and kxyz[d, :] at the cells either side (rhs/invscd.py:53-97).
-DF void face_flux_lr(const double pL[5], const double pR[5], const double k0[3], const double k1[3], double g,
-                     double Fa[5]) {
-  const double nf0 = 0.5 * (k0[0] + k1[0]), nf1 = 0.5 * (k0[1] + k1[1]), nf2 = 0.5 * (k0[2] + k1[2]);
+// primitives and the face normal nf = (kxyz[d, :] at the two cells) / 2
+// (rhs/invscd.py:53-97).
+DF void face_flux_n(const double pL[5], const double pR[5], double nf0, double nf1, double nf2, double g,
+                    double Fa[5]) {
   // |n| + 1e-300 and its reciprocal from one 1/sqrt (|n| = 0: 1e-300, 1e300)
   const double nn = nf0 * nf0 + nf1 * nf1 + nf2 * nf2;
   const double isn = rsq_acc(nn);
@@ -366,6 +367,20 @@ DF void face_flux_lr(const double pL[5], const double pR[5], const double k0[3],
   roe_flux_prim(pL, pR, nu, g, F);
 #pragma unroll
   for (int v = 0; v < 5; ++v) Fa[v] = F[v] * nm;
+}
+
+// ... from kxyz[d, :] at the cells either side
+DF void face_flux_lr(const double pL[5], const double pR[5], const double k0[3], const double k1[3], double g,
+                     double Fa[5]) {
+  face_flux_n(pL, pR, 0.5 * (k0[0] + k1[0]), 0.5 * (k0[1] + k1[1]), 0.5 * (k0[2] + k1[2]), g, Fa);
+}
+
+// ... from kxyz[d, :] / 2 at the cells either side: k0h + k1h is
+// 0.5 (k0 + k1) exactly (halving is exact and commutes with rounding), so
+// the same bits as face_flux_lr
+DF void face_flux_lr_h(const double pL[5], const double pR[5], const double k0h[3], const double k1h[3], double g,
+                       double Fa[5]) {
+  face_flux_n(pL, pR, k0h[0] + k1h[0], k0h[1] + k1h[1], k0h[2] + k1h[2], g, Fa);
 }
 
 

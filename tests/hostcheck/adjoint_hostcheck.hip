@@ -23,6 +23,11 @@ void hc_cons_to_prim_vjp(int n, const double* q, double g, const double* lp, dou
   }
 }
 
+// the stage kernels' minmod (famrdp_math.h minmod_fast)
+void hc_minmod_fast(int n, const double* a, const double* b, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = minmod_fast(a[i], b[i]);
+}
+
 void hc_limiter_vjp(int n, int lim, const double* A, const double* B, const double* ls, double* lA, double* lB) {
   for (int i = 0; i < n; ++i) limiter_vjp(lim, A[i], B[i], ls[i], lA[i], lB[i]);
 }

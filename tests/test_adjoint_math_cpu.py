@@ -81,6 +81,26 @@ def test_cons_to_prim_vjp(hc):
     assert _rel(lq, tq.grad.numpy().T) < 1e-12
 
 
+def test_minmod_fast_equals_reference_minmod(hc):
+    """The stage kernels' minmod (median of a, b and 0 in four min/max) has
+    the value of muscl2.py:5-7 for every finite or infinite pair; a NaN
+    argument is dropped (the other one is returned, two NaNs give 0) -- the
+    face routine re-injects the stencil's NaN itself."""
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=400); b = rng.normal(size=400)
+    b[:30] = a[:30]; b[30:60] = -a[30:60]; b[60:80] = 2 * a[60:80]
+    a[80:90] = 0.0; b[90:100] = -0.0; a[100:104] = [np.inf, -np.inf, np.inf, 1.0]
+    b[100:104] = [2.0, -3.0, -1.0, np.inf]
+    out = np.zeros(400)
+    hc.hc_minmod_fast(400, P(a), P(b), P(out))
+    np.testing.assert_array_equal(out, ON.minmod(a, b))
+    nan = np.nan
+    a2 = np.array([nan, 2.0, nan, nan, -1.5]); b2 = np.array([3.0, nan, -2.0, nan, nan])
+    out2 = np.zeros(5)
+    hc.hc_minmod_fast(5, P(a2), P(b2), P(out2))
+    np.testing.assert_array_equal(out2, [3.0, 2.0, -2.0, 0.0, -1.5])
+
+
 @pytest.mark.parametrize("lim", [0, 1])
 def test_limiter_vjp_with_ties_and_zeros(hc, lim):
     rng = np.random.default_rng(23 + lim)

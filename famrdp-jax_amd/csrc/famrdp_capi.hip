@@ -206,8 +206,10 @@ int upload_table(famrdp_plan* p, BlockDesc* dst, const std::vector<BlockDesc>& s
 // state pointers as they are (the reverse sweep reads its own copy, qh);
 // kxyz / vol == nullptr keep the bound metrics.  Tables are re-uploaded only
 // when a pointer changed.
+// q1t / q2t: where the step's stage-1/2 results go (caller arrays), or NULL
+// for the plan's workspace.
 int bind(famrdp_plan* p, double* const* q, const double* const* kxyz, const double* const* vol, hipStream_t st,
-         bool need_q = true) {
+         bool need_q = true, double* const* q1t = nullptr, double* const* q2t = nullptr) {
   if (need_q && !q) return fail(FAMRDP_E_INVALID, "q pointer array is NULL");
   bool changed = !p->blocks_uploaded, changed_adj = !p->blocks_uploaded;
   for (int b = 0; b < p->nblocks; ++b) {
@@ -215,11 +217,15 @@ int bind(famrdp_plan* p, double* const* q, const double* const* kxyz, const doub
     if (q && !q[b]) return fail(FAMRDP_E_INVALID, "q[" + std::to_string(b) + "] is NULL");
     const double* kx = kxyz ? kxyz[b] : d.kxyz;
     const double* vo = vol ? vol[b] : d.vol;
-    if ((q && d.q != q[b]) || d.kxyz != kx || d.vol != vo) changed = true;
+    double* s1 = q1t ? q1t[b] : p->q1[b];
+    double* s2 = q2t ? q2t[b] : p->q2[b];
+    if ((q && d.q != q[b]) || d.kxyz != kx || d.vol != vo || d.q1 != s1 || d.q2 != s2) changed = true;
     if (d.kxyz != kx || d.vol != vo) changed_adj = true;
     if (q) d.q = q[b];
     d.kxyz = kx;
     d.vol = vo;
+    d.q1 = s1;
+    d.q2 = s2;
   }
   if (changed) {
     int rc = upload_table(p, p->d_blocks, p->h_blocks, st);
@@ -883,8 +889,8 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
   for (int b = 0; b < p->nblocks; ++b) {
     BlockDesc& a = p->h_blocks_adj[b];
     double* qh = consume ? q_start[b] : p->adj_bufs[6 * b];   // adj_bufs: qh, la, lb, l0, fs, vs per block
-    double* q1 = q1s ? const_cast<double*>(q1s[b]) : p->h_blocks[b].q1;
-    double* q2 = q1s ? const_cast<double*>(q2s[b]) : p->h_blocks[b].q2;
+    double* q1 = q1s ? const_cast<double*>(q1s[b]) : p->q1[b];
+    double* q2 = q1s ? const_cast<double*>(q2s[b]) : p->q2[b];
     if (a.qh != qh || a.q != qh || a.q1 != q1 || a.q2 != q2 || a.la != lam[b]) rebind = true;
     a.qh = qh;
     a.q = qh;
@@ -966,10 +972,10 @@ int famrdp_plan_stage_states(famrdp_plan* p, double* const* q1_out, double* cons
   for (int b = 0; b < p->nblocks; ++b)
     if (!q1_out[b] || !q2_out[b]) return fail(FAMRDP_E_INVALID, "stage_states: NULL pointer for block " + std::to_string(b));
   int rc = copy_blocks(p, st, [&](int b) { return q1_out[b]; },
-                       [&](int b) { return static_cast<const double*>(p->h_blocks[b].q1); });
+                       [&](int b) { return static_cast<const double*>(p->q1[b]); });
   if (rc) return rc;
   return copy_blocks(p, st, [&](int b) { return q2_out[b]; },
-                     [&](int b) { return static_cast<const double*>(p->h_blocks[b].q2); });
+                     [&](int b) { return static_cast<const double*>(p->q2[b]); });
 }
 
 int famrdp_objective(int32_t kind, const double* q, const double* kxyz, const double* vol, int32_t ni, int32_t nj,
@@ -1017,6 +1023,18 @@ int famrdp_step(famrdp_plan* p, double* const* q, const double* const* kxyz, con
   for (int b = 0; b < p->nblocks; ++b)
     if (!kxyz[b] || !vol[b]) return fail(FAMRDP_E_INVALID, "step: NULL kxyz/vol for block " + std::to_string(b));
   int rc = bind(p, q, kxyz, vol, S(stream));
+  if (rc) return rc;
+  return run_step(p, dt, flags, S(stream));
+}
+
+int famrdp_step_stages(famrdp_plan* p, double* const* q, const double* const* kxyz, const double* const* vol,
+                       double dt, int32_t flags, double* const* q1_out, double* const* q2_out, void* stream) {
+  if (!p || !kxyz || !vol || !q1_out || !q2_out) return fail(FAMRDP_E_INVALID, "step_stages: NULL plan or pointer arrays");
+  if (!p->has_workspace) return fail(FAMRDP_E_INVALID, "step_stages: plan was created without stage workspace");
+  for (int b = 0; b < p->nblocks; ++b)
+    if (!kxyz[b] || !vol[b] || !q1_out[b] || !q2_out[b])
+      return fail(FAMRDP_E_INVALID, "step_stages: NULL kxyz/vol/q1/q2 for block " + std::to_string(b));
+  int rc = bind(p, q, kxyz, vol, S(stream), true, q1_out, q2_out);
   if (rc) return rc;
   return run_step(p, dt, flags, S(stream));
 }

@@ -65,6 +65,8 @@ SIGNATURES = {
     "famrdp_plan_destroy": ([vp], i32),
     "famrdp_step": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, i32, vp], i32),
     "famrdp_step_graph": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, i32, i32, vp], i32),
+    "famrdp_step_stages": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, i32, ctypes.POINTER(vp),
+                            ctypes.POINTER(vp), vp], i32),
     "famrdp_plan_autotune": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), f64, i32, i32,
                               ctypes.POINTER(f64), vp], i32),
     "famrdp_plan_set_tile": ([vp, i32], i32),

@@ -106,12 +106,20 @@ class Plan:
     def _ptrs(ts):
         return (L.vp * len(ts))(*[L.ptr(t) for t in ts])
 
-    def step(self, qs, kxyzs, vols, dt, *, batched=False, graph_steps=0, direct=False):
+    def step(self, qs, kxyzs, vols, dt, *, batched=False, graph_steps=0, direct=False, stages=None):
         """One step in place on the device tensors ``qs`` (or ``graph_steps``
         steps replayed from a captured hipGraph).  ``direct`` selects the
-        reference-order one-thread-per-cell kernels instead of the tiled ones."""
+        reference-order one-thread-per-cell kernels instead of the tiled ones.
+        ``stages`` = (q1s, q2s), per-block device tensors shaped like ``qs``:
+        the step writes its stage-1/2 states there (eager steps only)."""
         flags = (1 if batched else 0) | (2 if direct else 0)
-        if graph_steps:
+        if stages is not None:
+            if graph_steps:
+                raise ValueError("stages: eager steps only")
+            q1s, q2s = stages
+            L.check(self._lib.famrdp_step_stages(self._h, self._ptrs(qs), self._ptrs(kxyzs), self._ptrs(vols),
+                                                 float(dt), flags, self._ptrs(q1s), self._ptrs(q2s), L.stream()))
+        elif graph_steps:
             L.check(self._lib.famrdp_step_graph(self._h, self._ptrs(qs), self._ptrs(kxyzs), self._ptrs(vols),
                                                 float(dt), flags, int(graph_steps), L.stream()))
         else:

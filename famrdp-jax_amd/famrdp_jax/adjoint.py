@@ -55,6 +55,14 @@ class Adjoint:
     def _step(self, qs, dt):
         self.plan.step(qs, self.kxyz, self.vol, dt, batched=self.batched, direct=self.direct)
 
+    def _step_stages(self, qs, dt):
+        """_step that also returns the step's stage-1/2 states (new tensors the
+        step writes them into)."""
+        q1s = [torch_empty_like(q) for q in qs]
+        q2s = [torch_empty_like(q) for q in qs]
+        self.plan.step(qs, self.kxyz, self.vol, dt, batched=self.batched, direct=self.direct, stages=(q1s, q2s))
+        return q1s, q2s
+
     def forward(self, dt, nsteps, qs=None):
         qs = [q.clone() for q in (qs or self.q0)]
         for _ in range(nsteps):
@@ -75,12 +83,9 @@ class Adjoint:
             keep = keep_last_segment and n >= last_start
             if keep:
                 starts.append([q.clone() for q in cur])
-            self._step(cur, dt)
-            if keep:
-                q1s = [torch_empty_like(c) for c in cur]
-                q2s = [torch_empty_like(c) for c in cur]
-                self.plan.stage_states(q1s, q2s)
-                stages.append((q1s, q2s))
+                stages.append(self._step_stages(cur, dt))
+            else:
+                self._step(cur, dt)
             if (n + 1) % checkpoint_every == 0 and n + 1 < nsteps:
                 checkpoints[n + 1] = [q.clone() for q in cur]
         if keep_last_segment:
@@ -120,12 +125,10 @@ class Adjoint:
                 starts, stages = [], []
                 for _ in range(seg_start, seg_end):
                     starts.append([c.clone() for c in cur])
-                    self._step(cur, dt)
                     if keep_stages:
-                        q1s = [torch_empty_like(c) for c in cur]
-                        q2s = [torch_empty_like(c) for c in cur]
-                        self.plan.stage_states(q1s, q2s)
-                        stages.append((q1s, q2s))
+                        stages.append(self._step_stages(cur, dt))
+                    else:
+                        self._step(cur, dt)
             for n in reversed(range(seg_end - seg_start)):
                 self.plan.step_vjp(starts[n], self.kxyz, self.vol, lam, dt, dqinf, batched=self.batched,
                                    direct=self.direct, stages=stages[n] if keep_stages else None,
@@ -149,11 +152,7 @@ class Adjoint:
             starts, stages = [], []
             for _ in range(nsteps):
                 starts.append([q.clone() for q in cur])
-                self._step(cur, dt)
-                q1s = [torch.empty_like(q) for q in cur]
-                q2s = [torch.empty_like(q) for q in cur]
-                self.plan.stage_states(q1s, q2s)
-                stages.append((q1s, q2s))
+                stages.append(self._step_stages(cur, dt))
             J, lam = self.objective(cur, kind, blocks)
             t1 = time.perf_counter()
             for n in reversed(range(nsteps)):

@@ -137,6 +137,14 @@ int famrdp_step(famrdp_plan* plan, double* const* q, const double* const* kxyz, 
  * flags (captured on first use, re-captured when any of them changes). */
 int famrdp_step_graph(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                       double dt, int32_t flags, int32_t nsteps, void* stream);
+/* famrdp_step that writes the step's stage-1 and stage-2 states (q1 and q2
+ * of solver/step.py:45-53, locals there) into q1_out / q2_out (per block,
+ * device 5*N doubles, distinct from q and from each other) instead of the
+ * plan's workspace: the states famrdp_step_vjp_stages takes, without a copy.
+ * Same step and bits as famrdp_step; famrdp_plan_stage_states keeps
+ * returning the workspace of the last famrdp_step / famrdp_step_graph. */
+int famrdp_step_stages(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
+                       double dt, int32_t flags, double* const* q1_out, double* const* q2_out, void* stream);
 /* Stage-kernel selection (no counterpart in the reference).  Variants 0-4
  * and 10-13 are tile shapes of the fused stage kernel (4 and 10-13 stage the
  * primitives twice to need less LDS), 5-9 tile shapes of the split pair
@@ -186,9 +194,9 @@ int famrdp_step_vjp(famrdp_plan* plan, double* const* q_start, const double* con
 int famrdp_step_vjp_stages(famrdp_plan* plan, double* const* q_start, const double* const* q1, const double* const* q2,
                            const double* const* kxyz, const double* const* vol, double* const* lam, double dt,
                            double* dqinf, int32_t flags, void* stream);
-/* Copies the stage-1 and stage-2 states of the plan's last famrdp_step (its
- * workspace) into q1_out / q2_out (per block, device 5*N doubles).  No
- * counterpart in the reference. */
+/* Copies the stage-1 and stage-2 states of the plan's last famrdp_step or
+ * famrdp_step_graph (its workspace) into q1_out / q2_out (per block, device
+ * 5*N doubles).  No counterpart in the reference. */
 int famrdp_plan_stage_states(famrdp_plan* plan, double* const* q1_out, double* const* q2_out, void* stream);
 
 /* Build-defined objectives for the differentiation workloads (the reference

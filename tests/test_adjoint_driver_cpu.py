@@ -18,11 +18,13 @@ class _Plan:
         self.vjps = []
         self._last = None
 
-    def step(self, qs, kxyz, vol, dt, batched=False, direct=False):
+    def step(self, qs, kxyz, vol, dt, batched=False, direct=False, stages=None):
         for q in qs:
             q += 1.0
         self._last = [q.clone() for q in qs]
         self.steps += 1
+        if stages is not None:   # the step writes its stage states into the caller's arrays
+            self.stage_states(*stages)
 
     def stage_states(self, q1s, q2s):
         # stage states of the step that just ran (ending at state n + 1): tags n + 1.25, n + 1.5

@@ -259,6 +259,37 @@ def test_step_vjp_consumed_or_copied_start_same_bits(batched):
     assert np.isfinite(outs[0][0]).all() and np.abs(outs[0][0]).max() > 0
 
 
+@pytest.mark.parametrize("batched", [False, True])
+def test_step_stages_writes_the_stage_states(batched):
+    """famrdp_step_stages gives the same step, bit for bit, as famrdp_step and
+    writes into the caller's arrays the stage states famrdp_plan_stage_states
+    copies out after famrdp_step; a famrdp_step after it (back on the plan's
+    workspace) is unchanged too, and stage_states still returns its states."""
+    print(f"case: C4-like 12^3, famrdp_step_stages vs famrdp_step, batched={batched}", flush=True)
+    state, cfg, dt = _c4_small(12)
+    adj = Adjoint(to_dev(state), cfg, batched=batched)
+    a = [q.clone() for q in adj.q0]
+    adj._step(a, dt)
+    r1 = [torch.empty_like(q) for q in a]
+    r2 = [torch.empty_like(q) for q in a]
+    adj.plan.stage_states(r1, r2)
+    b = [q.clone() for q in adj.q0]
+    q1s, q2s = adj._step_stages(b, dt)
+    torch.cuda.synchronize()
+    for x, y in ((a, b), (r1, q1s), (r2, q2s)):
+        assert torch.equal(x[0], y[0])
+    assert not torch.equal(q1s[0], q2s[0]) and not torch.equal(q1s[0], b[0])
+    # the next plain step runs on the workspace again: same bits both ways
+    adj._step(a, dt)
+    adj._step(b, dt)
+    s1 = [torch.empty_like(q) for q in a]
+    s2 = [torch.empty_like(q) for q in a]
+    adj.plan.stage_states(s1, s2)
+    torch.cuda.synchronize()
+    assert torch.equal(a[0], b[0])
+    assert not torch.equal(s1[0], q1s[0]) and torch.isfinite(s1[0]).all()
+
+
 def test_torch_autograd_function():
     print("case: torch.autograd.Function wrapper", flush=True)
     st, cfg = cases.case_c1(n=8, seed=9, p_noise=1e-3)

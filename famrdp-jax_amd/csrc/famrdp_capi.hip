@@ -603,7 +603,7 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     if ((int64_t)d[0] * d[1] * d[2] > INT32_MAX)   // tile kernels index cells in 32 bits
       return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": 2^31 or more cells"));
     // the reverse kernels address cells with 32-bit byte offsets; an adjoint
-    // plan allocates 76 doubles per cell (stage and adjoint buffers), so such a
+    // plan allocates 80 doubles per cell (stage and adjoint buffers), so such a
     // block would need over 300 GB of them anyway
     if ((plan_flags & FAMRDP_PLAN_ADJOINT) && (int64_t)d[0] * d[1] * d[2] >= (int64_t(1) << 29))
       return cleanup(fail(FAMRDP_E_INVALID, "block " + std::to_string(b) + ": 2^29 or more cells in an adjoint plan"));
@@ -638,9 +638,15 @@ int famrdp_plan_create(famrdp_plan** out, int32_t nblocks, const int32_t* dims, 
     p->h_blocks_adj = p->h_blocks;
     for (int b = 0; b < nblocks; ++b) {
       const int64_t n = p->N[b];
-      const int64_t sizes[6] = {5 * n, 5 * n, 5 * n, 5 * n, 30 * n, 16 * n};
+      // qh, la, lb, (l0: the caller's cotangent array, bound by each
+      // famrdp_step_vjp), fs, vs
+      const int64_t sizes[6] = {5 * n, 5 * n, 5 * n, 0, 30 * n, 16 * n};
       double* bufs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
       for (int i = 0; i < 6; ++i) {
+        if (sizes[i] == 0) {
+          p->adj_bufs.push_back(nullptr);
+          continue;
+        }
         if (hipMalloc(&bufs[i], sizeof(double) * sizes[i]) != hipSuccess)
           return cleanup(fail(FAMRDP_E_HIP, "plan_create: adjoint workspace allocation failed"));
         p->adj_bufs.push_back(bufs[i]);
@@ -881,22 +887,26 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
   if (rc) return rc;
   // The adjoint table reads the caller's arrays where it can instead of
   // copies of them: the stage states when supplied (read only), lam as the
-  // cotangent entering stage 3 (stage 2 overwrites it with its own, and the
-  // result is written to it at the end), q_start itself as the state the
-  // halo and BCs are applied to when the caller gives it up.
+  // cotangent of the post-halo/BC state l0 -- it enters stage 3 there, which
+  // scales it in place (each cell reads and then writes its own), stages 2
+  // and 1 add to it and the BC and halo reverses finish the result in it --
+  // and q_start itself as the state the halo and BCs are applied to when the
+  // caller gives it up.
   const bool consume = (flags & FAMRDP_VJP_CONSUME_START) != 0;
   bool rebind = false;
   for (int b = 0; b < p->nblocks; ++b) {
     BlockDesc& a = p->h_blocks_adj[b];
-    double* qh = consume ? q_start[b] : p->adj_bufs[6 * b];   // adj_bufs: qh, la, lb, l0, fs, vs per block
+    double* qh = consume ? q_start[b] : p->adj_bufs[6 * b];   // adj_bufs: qh, la, lb, (l0), fs, vs per block
     double* q1 = q1s ? const_cast<double*>(q1s[b]) : p->q1[b];
     double* q2 = q1s ? const_cast<double*>(q2s[b]) : p->q2[b];
-    if (a.qh != qh || a.q != qh || a.q1 != q1 || a.q2 != q2 || a.la != lam[b]) rebind = true;
+    double* la = p->adj_bufs[6 * b + 1];
+    if (a.qh != qh || a.q != qh || a.q1 != q1 || a.q2 != q2 || a.la != la || a.l0 != lam[b]) rebind = true;
     a.qh = qh;
     a.q = qh;
     a.q1 = q1;
     a.q2 = q2;
-    a.la = lam[b];
+    a.la = la;
+    a.l0 = lam[b];
   }
   if (rebind) {
     rc = upload_table(p, p->d_blocks_adj, p->h_blocks_adj, st);
@@ -943,9 +953,6 @@ int step_vjp_impl(famrdp_plan* p, double* const* q_start, const double* const* q
     k_halo_vjp_apply<<<nblk(p->nhalo_src), 256, 0, st>>>(p->d_blocks_adj, p->d_halo_src, p->nhalo_src, p->d_halo_S);
     LAUNCH_CHECK();
   }
-  rc = copy_blocks(p, st, [&](int b) { return lam[b]; },
-                   [&](int b) { return static_cast<const double*>(p->h_blocks_adj[b].l0); });
-  if (rc) return rc;
   return FAMRDP_OK;
 }
 

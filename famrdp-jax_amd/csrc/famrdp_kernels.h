@@ -27,7 +27,7 @@ struct BlockDesc {
   double* qh;           // post-halo/BC state recomputed for the reverse sweep (5, N)
   double* la;           // stage cotangents, ping (5, N)
   double* lb;           // stage cotangents, pong (5, N)
-  double* l0;           // cotangent of the post-halo/BC state (5, N)
+  double* l0;           // cotangent of the post-halo/BC state (5, N): the caller's cotangent array
   double* fs;           // inviscid face reverse scratch (3 axes x 10, N)
   double* vs;           // viscous reverse scratch (16, N)
 };

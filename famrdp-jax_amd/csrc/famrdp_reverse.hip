@@ -25,7 +25,7 @@
 namespace famrdp {
 
 // Cells are addressed with 32-bit element indices (an adjoint plan's blocks
-// hold fewer than 2^29 cells: famrdp_plan_create refuses larger ones, whose 76
+// hold fewer than 2^29 cells: famrdp_plan_create refuses larger ones, whose 80
 // doubles per cell of stage and adjoint buffers would exceed 300 GB), loads and stores through a 32-bit byte offset from
 // a base that is the same for the whole workgroup -- the base stays in scalar
 // registers and a cell costs one 32-bit offset, however many arrays it reads
@@ -159,7 +159,8 @@ __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* 
 
 // ---------------------------------------------------------------------------
 // per-cell gather, primitive -> conservative, RK coefficients
-//   mode 0: l0 = c0 lin,  lout = c_next lin + g     (stage 3)
+//   mode 0: l0 = c0 lin,  lout = c_next lin + g     (stage 3; lin is l0, each
+//           cell's value read before it is written)
 //   mode 1: l0 += c0 lin, lout = c_next lin + g     (stage 2)
 //   mode 2: l0 += lin + g                           (stage 1)
 // ---------------------------------------------------------------------------
@@ -245,13 +246,14 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
 }
 
 // Stage selector for the reverse sweep: which buffers a stage reads/writes.
-//   stage 3: qs = q2, lin = la (cotangent of q_new), lout = lb
+//   stage 3: qs = q2, lin = l0 (the cotangent of q_new, which the gather
+//            replaces cell by cell with its q0 share: l0 = lin / 3), lout = lb
 //   stage 2: qs = q1, lin = lb, lout = la
 //   stage 1: qs = qh, lin = la
 __device__ __forceinline__ void vjp_bufs(const BlockDesc& b, int stage, const double*& qs, const double*& lin,
                                          double*& lout) {
   qs = stage == 3 ? b.q2 : (stage == 2 ? b.q1 : b.qh);
-  lin = stage == 3 ? b.la : (stage == 2 ? b.lb : b.la);
+  lin = stage == 3 ? b.l0 : (stage == 2 ? b.lb : b.la);
   lout = stage == 3 ? b.lb : (stage == 2 ? b.la : nullptr);
 }
 

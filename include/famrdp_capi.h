@@ -178,7 +178,9 @@ int famrdp_plan_set_block_params(famrdp_plan* plan, const double* mu0, const dou
 /* Reverse of one famrdp_step (solver/step.py:18-56 under jax.grad, see
  * tests/solver/test_grad.py:10-38): given q_start (the state the step started
  * from, per block) and lam (cotangent of the step's result, per block, device
- * 5*N doubles), overwrites lam with the cotangent of q_start.  Recomputes
+ * 5*N doubles), overwrites lam with the cotangent of q_start (lam is the
+ * sweep's accumulator from the start: the caller must not read it before the
+ * stream reaches the end of the call).  Recomputes
  * halo, BCs and stages 1-2, reverses the three stages (viscous, Roe, MUSCL),
  * the BCs in reverse Face order and the halo copy.  dqinf (device, NULL or
  * nblocks*6*5 doubles) accumulates the cotangent of each (block, face)
@@ -187,8 +189,9 @@ int famrdp_plan_set_block_params(famrdp_plan* plan, const double* mu0, const dou
 int famrdp_step_vjp(famrdp_plan* plan, double* const* q_start, const double* const* kxyz, const double* const* vol,
                     double* const* lam, double dt, double* dqinf, int32_t flags, void* stream);
 /* famrdp_step_vjp with the step's stage-1 and stage-2 states supplied (per
- * block, device 5*N doubles, as saved by famrdp_plan_stage_states right after
- * the forward famrdp_step from q_start) instead of recomputed.  Same reverse
+ * block, device 5*N doubles, as written by famrdp_step_stages or saved by
+ * famrdp_plan_stage_states right after the forward famrdp_step from q_start)
+ * instead of recomputed.  Same reverse
  * sweep and results; no counterpart in the reference (checkpointed reverse
  * passes that keep a segment's stage states). */
 int famrdp_step_vjp_stages(famrdp_plan* plan, double* const* q_start, const double* const* q1, const double* const* q2,

@@ -1038,9 +1038,12 @@ int famrdp_step_stages(famrdp_plan* p, double* const* q, const double* const* kx
                        double dt, int32_t flags, double* const* q1_out, double* const* q2_out, void* stream) {
   if (!p || !kxyz || !vol || !q1_out || !q2_out) return fail(FAMRDP_E_INVALID, "step_stages: NULL plan or pointer arrays");
   if (!p->has_workspace) return fail(FAMRDP_E_INVALID, "step_stages: plan was created without stage workspace");
-  for (int b = 0; b < p->nblocks; ++b)
+  for (int b = 0; b < p->nblocks; ++b) {
     if (!kxyz[b] || !vol[b] || !q1_out[b] || !q2_out[b])
       return fail(FAMRDP_E_INVALID, "step_stages: NULL kxyz/vol/q1/q2 for block " + std::to_string(b));
+    if (q && (q1_out[b] == q[b] || q2_out[b] == q[b] || q1_out[b] == q2_out[b]))
+      return fail(FAMRDP_E_INVALID, "step_stages: q, q1 and q2 of block " + std::to_string(b) + " must be distinct");
+  }
   int rc = bind(p, q, kxyz, vol, S(stream), true, q1_out, q2_out);
   if (rc) return rc;
   return run_step(p, dt, flags, S(stream));

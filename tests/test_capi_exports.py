@@ -63,6 +63,16 @@ def test_plan_create_rejects_bad_block_sizes_before_allocating():
         assert msg in L.famrdp_last_error()
 
 
+def test_step_entry_points_refuse_null_plan():
+    """The step entry points check their arguments before any HIP call."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built (run __graft_entry__.build())")
+    L = _lib.load()
+    assert L.famrdp_step(None, None, None, None, 0.1, 0, None) == _lib.E_INVALID
+    assert L.famrdp_step_stages(None, None, None, None, 0.1, 0, None, None, None) == _lib.E_INVALID
+    assert b"step_stages" in L.famrdp_last_error()
+
+
 def test_float64_guard():
     import numpy as np
     with pytest.raises(TypeError):

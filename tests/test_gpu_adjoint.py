@@ -288,6 +288,11 @@ def test_step_stages_writes_the_stage_states(batched):
     torch.cuda.synchronize()
     assert torch.equal(a[0], b[0])
     assert not torch.equal(s1[0], q1s[0]) and torch.isfinite(s1[0]).all()
+    # the stage arrays must be distinct from the state and from each other
+    with pytest.raises(ValueError):
+        adj.plan.step(b, adj.kxyz, adj.vol, dt, batched=batched, stages=(b, q2s))
+    with pytest.raises(ValueError):
+        adj.plan.step(b, adj.kxyz, adj.vol, dt, batched=batched, stages=(q1s, q1s))
 
 
 def test_torch_autograd_function():

@@ -424,6 +424,14 @@ int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, do
   return FAMRDP_OK;
 }
 
+int famrdp_fastmath_probe1(const double* x, double* out_rcp, double* out_rsqrt, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !out_rcp || !out_rsqrt))) return fail(FAMRDP_E_INVALID, "fastmath_probe1: bad arguments");
+  if (n == 0) return FAMRDP_OK;
+  k_fastmath_probe1<<<nblk(n), 256, 0, S(stream)>>>(x, out_rcp, out_rsqrt, n);
+  LAUNCH_CHECK();
+  return FAMRDP_OK;
+}
+
 int famrdp_muscl2(const double* pv, double* pv_L, double* pv_R, int32_t nvar, int32_t n0, int32_t n1, int32_t n2,
                   int32_t axis, int32_t limiter, void* stream) {
   if (bad_lim(limiter)) return fail(FAMRDP_E_KEY, "unknown limiter code " + std::to_string(limiter));

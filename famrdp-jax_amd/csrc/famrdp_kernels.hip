@@ -247,6 +247,15 @@ __global__ void k_fastmath_probe(const double* __restrict__ x, double* __restric
   }
 }
 
+__global__ void k_fastmath_probe1(const double* __restrict__ x, double* __restrict__ r, double* __restrict__ q,
+                                  int64_t n) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) {
+    r[t] = rcp1(x[t]);
+    q[t] = rsq1(x[t]);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_stage(const double* __restrict__ qin, const double* q0,
                                                double* qout, const double* __restrict__ kxyz,
                                                const double* __restrict__ vol, const double* __restrict__ G, int ni,

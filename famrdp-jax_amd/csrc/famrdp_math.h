@@ -232,6 +232,18 @@ DF double rsq_acc(double x) {
   return 2.0 * h;
 }
 
+// One Newton step on the hardware seeds (relative error 2^-24 on gfx950, so
+// about 2^-47 after the step: at most ~20 ulp, measured on the GPU), for the
+// face flux, whose every quotient and root enters the flux at O(1) weight.
+DF double rcp1(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+DF double rsq1(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return fma(y * fma(-(x * y), y, 1.0), 0.5, y);
+}
+
 // van Leer with the division as a reciprocal (stage-kernel form)
 DF double vanleer_fast(double a, double b) {
   const double ab = a + b;
@@ -265,27 +277,27 @@ DF void roe_flux_prim(const double pl[5], const double pr[5], const double n[3],
   const double EL = pL * rgm1 + rhoL * (0.5 * (uL * uL + vL * vL + wL * wL));
   const double ER = pR * rgm1 + rhoR * (0.5 * (uR * uR + vR * vR + wR * wR));
   // 1/sqrt(rho) gives sqrt(rho), 1/rho and, below, 1/sqrt(rhoL rhoR)
-  const double isL = rsq_acc(rhoL), isR = rsq_acc(rhoR);
+  const double isL = rsq1(rhoL), isR = rsq1(rhoR);
   const double HL = (EL + pL) * (isL * isL), HR = (ER + pR) * (isR * isR);
 
   const double sqL = rhoL * isL, sqR = rhoR * isR;
   // max(sqL + sqR, 1e-150) is sqL + sqR itself: both roots are >= 1e-150
   // after the 1e-300 floors (and NaN stays NaN), so the floor never acts
-  const double rs = rcp(sqL + sqR);
+  const double rs = rcp1(sqL + sqR);
   const double uh = (sqL * uL + sqR * uR) * rs;
   const double vh = (sqL * vL + sqR * vR) * rs;
   const double wh = (sqL * wL + sqR * wR) * rs;
   const double Hh = (sqL * HL + sqR * HR) * rs;
   const double q2 = uh * uh + vh * vh + wh * wh;
   const double c2 = max_floor(gm1 * (Hh - 0.5 * q2), 1e-30);
-  const double isc = rsq_acc(c2);
+  const double isc = rsq1(c2);
   const double rc2 = isc * isc;
   const double ch = c2 * isc;
   const double unh = uh * n[0] + vh * n[1] + wh * n[2];
   const double rhoh = sqL * sqR;
 
   const double eps = 0.1 * (fabs(unh) + ch);
-  const double reps = rcp(eps);
+  const double reps = rcp1(eps);
   const double lam1 = unh - ch, lam5 = unh + ch;
   const double al1 = fabs(lam1) >= eps ? fabs(lam1) : 0.5 * (lam1 * lam1 * reps + eps);
   const double al5 = fabs(lam5) >= eps ? fabs(lam5) : 0.5 * (lam5 * lam5 * reps + eps);
@@ -358,7 +370,7 @@ DF void face_flux_n(const double pL[5], const double pR[5], double nf0, double n
                     double Fa[5]) {
   // |n| + 1e-300 and its reciprocal from one 1/sqrt (|n| = 0: 1e-300, 1e300)
   const double nn = nf0 * nf0 + nf1 * nf1 + nf2 * nf2;
-  const double isn = rsq_acc(nn);
+  const double isn = rsq1(nn);
   const bool pos = nn > 0.0;
   const double nm = (pos ? nn * isn : 0.0) + 1e-300;
   const double rn = pos ? isn : 1e300;

@@ -54,6 +54,7 @@ SIGNATURES = {
     "famrdp_speed_of_sound": ([vp, vp, vp, i64, f64, vp], i32),
     "famrdp_limiter": ([vp, vp, vp, i64, i32, vp], i32),
     "famrdp_fastmath_probe": ([vp, vp, vp, vp, i64, vp], i32),
+    "famrdp_fastmath_probe1": ([vp, vp, vp, i64, vp], i32),
     "famrdp_muscl2": ([vp, vp, vp, i32, i32, i32, i32, i32, i32, vp], i32),
     "famrdp_roe_flux": ([vp, vp, vp, vp, i64, f64, f64, vp], i32),
     "famrdp_invscd_rhs": ([vp, vp, vp, vp, i32, i32, i32, f64, i32, vp], i32),

@@ -106,6 +106,10 @@ int famrdp_compute_metrics(const double* xyz, double* jac, double* kxyz, double*
  * No counterpart in the reference. */
 int famrdp_fastmath_probe(const double* x, double* out_rcp, double* out_sqrt, double* out_rsqrt, int64_t n,
                           void* stream);
+/* The face flux's reciprocal and reciprocal square root (hardware seed + one
+ * Newton step) on n values, for verifying their accuracy against IEEE 1/x
+ * and 1/sqrt(x) (x > 0).  No counterpart in the reference. */
+int famrdp_fastmath_probe1(const double* x, double* out_rcp, double* out_rsqrt, int64_t n, void* stream);
 
 /* ---- plan: one multi-block configuration (Config.topology + gas + scheme) - */
 

@@ -260,3 +260,12 @@ def test_fastmath_rcp_sqrt_accuracy():
     r0 = torch.empty_like(z); s0 = torch.empty_like(z)
     L.check(L.lib().famrdp_fastmath_probe(z.data_ptr(), r0.data_ptr(), s0.data_ptr(), None, 4, L.stream()))
     assert (s0.cpu().numpy() == 0.0).all()
+    # the face flux's one-step forms: a few tens of ulp (hardware seeds are good
+    # to 2^-24, one Newton step squares that)
+    r1 = torch.empty_like(xd); q1 = torch.empty_like(xd)
+    L.check(L.lib().famrdp_fastmath_probe1(xd.data_ptr(), r1.data_ptr(), q1.data_ptr(), x.size, L.stream()))
+    r1 = r1.cpu().numpy(); q1 = q1.cpu().numpy()
+    e1 = (np.abs(r1 - 1.0 / x) / ulp).max()
+    eq1 = (np.abs(q1 - rq) / np.spacing(rq)).max()
+    print(f"  one-step rcp max error {e1:.2f} ulp, rsqrt {eq1:.2f} ulp", flush=True)
+    assert e1 <= 32.0 and eq1 <= 32.0

@@ -288,6 +288,17 @@ def test_step_stages_writes_the_stage_states(batched):
     torch.cuda.synchronize()
     assert torch.equal(a[0], b[0])
     assert not torch.equal(s1[0], q1s[0]) and torch.isfinite(s1[0]).all()
+    # a captured graph around it keeps stepping on the workspace: graph, stages, graph
+    # steps give the bits of three plain steps
+    c = [q.clone() for q in adj.q0]
+    d = [q.clone() for q in adj.q0]
+    for _ in range(3):
+        adj._step(c, dt)
+    adj.plan.step(d, adj.kxyz, adj.vol, dt, batched=batched, graph_steps=1)
+    adj._step_stages(d, dt)
+    adj.plan.step(d, adj.kxyz, adj.vol, dt, batched=batched, graph_steps=1)
+    torch.cuda.synchronize()
+    assert torch.equal(c[0], d[0])
     # the stage arrays must be distinct from the state and from each other
     with pytest.raises(ValueError):
         adj.plan.step(b, adj.kxyz, adj.vol, dt, batched=batched, stages=(b, q2s))

@@ -259,15 +259,15 @@ def test_step_vjp_consumed_or_copied_start_same_bits(batched):
     assert np.isfinite(outs[0][0]).all() and np.abs(outs[0][0]).max() > 0
 
 
-@pytest.mark.parametrize("batched", [False, True])
-def test_step_stages_writes_the_stage_states(batched):
+@pytest.mark.parametrize("batched,direct", [(False, False), (True, False), (False, True)])
+def test_step_stages_writes_the_stage_states(batched, direct):
     """famrdp_step_stages gives the same step, bit for bit, as famrdp_step and
     writes into the caller's arrays the stage states famrdp_plan_stage_states
     copies out after famrdp_step; a famrdp_step after it (back on the plan's
     workspace) is unchanged too, and stage_states still returns its states."""
-    print(f"case: C4-like 12^3, famrdp_step_stages vs famrdp_step, batched={batched}", flush=True)
+    print(f"case: C4-like 12^3, famrdp_step_stages vs famrdp_step, batched={batched}, direct={direct}", flush=True)
     state, cfg, dt = _c4_small(12)
-    adj = Adjoint(to_dev(state), cfg, batched=batched)
+    adj = Adjoint(to_dev(state), cfg, batched=batched, direct=direct)
     a = [q.clone() for q in adj.q0]
     adj._step(a, dt)
     r1 = [torch.empty_like(q) for q in a]
@@ -294,16 +294,16 @@ def test_step_stages_writes_the_stage_states(batched):
     d = [q.clone() for q in adj.q0]
     for _ in range(3):
         adj._step(c, dt)
-    adj.plan.step(d, adj.kxyz, adj.vol, dt, batched=batched, graph_steps=1)
+    adj.plan.step(d, adj.kxyz, adj.vol, dt, batched=batched, direct=direct, graph_steps=1)
     adj._step_stages(d, dt)
-    adj.plan.step(d, adj.kxyz, adj.vol, dt, batched=batched, graph_steps=1)
+    adj.plan.step(d, adj.kxyz, adj.vol, dt, batched=batched, direct=direct, graph_steps=1)
     torch.cuda.synchronize()
     assert torch.equal(c[0], d[0])
     # the stage arrays must be distinct from the state and from each other
     with pytest.raises(ValueError):
-        adj.plan.step(b, adj.kxyz, adj.vol, dt, batched=batched, stages=(b, q2s))
+        adj.plan.step(b, adj.kxyz, adj.vol, dt, batched=batched, direct=direct, stages=(b, q2s))
     with pytest.raises(ValueError):
-        adj.plan.step(b, adj.kxyz, adj.vol, dt, batched=batched, stages=(q1s, q1s))
+        adj.plan.step(b, adj.kxyz, adj.vol, dt, batched=batched, direct=direct, stages=(q1s, q1s))
 
 
 def test_torch_autograd_function():

@@ -47,8 +47,11 @@ __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double*
   const int64_t N = b.N;
   const unsigned njk = (unsigned)nj * (unsigned)nk;
   const int k = (int)(a % (unsigned)nk), j = (int)((a / (unsigned)nk) % (unsigned)nj), i = (int)(a / njk);
+  // axes k, j, i: the viscous part that runs first in the same thread ends
+  // with the k-neighbours, so the k faces find them in cache; each axis writes
+  // only its own face slots, so the order leaves every result unchanged
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 2; d >= 0; --d) {
     const int ad = d == 0 ? i : (d == 1 ? j : k);
     const int nd = d == 0 ? ni : (d == 1 ? nj : nk);
     const unsigned sd = d == 0 ? njk : (d == 1 ? (unsigned)nk : 1u);

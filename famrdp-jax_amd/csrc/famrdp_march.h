@@ -142,6 +142,29 @@ DF void march_face(const double* sP, double* sF, int base, int str, __amdgpu_buf
   for (int v = 0; v < 5; ++v) sF[v * M::NFB + f] = Fa[v];
 }
 
+// A thread's own low j- or k-face (rounds 0 and 1): the right-hand cell is
+// the thread's column, whose kxyz / 2 of the plane (krh, rows s0 / nb ..) the
+// thread holds in registers; the left-hand row (plane offset cl) is loaded
+// and halved, which is exact, so the face has the bits of march_face.
+template <int TJ, int TK, int LIM>
+DF void march_face_own(const double* sP, double* sF, int base, int str, __amdgpu_buffer_rsrc_t rk, unsigned cl,
+                       unsigned s0, unsigned nb, const double krh[3], int f, double g) {
+  using M = MarchGeom<TJ, TK>;
+  double p4[4][5];
+#pragma unroll
+  for (int s4 = 0; s4 < 4; ++s4) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) p4[s4][v] = sP[v * M::NP + base + s4 * str];
+  }
+  double klh[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) klh[m] = 0.5 * ldx(rk, cl, s0 + m * nb);
+  double Fa[5];
+  face_flux_prim_h<LIM>(p4[0], p4[1], p4[2], p4[3], klh, krh, g, Fa);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sF[v * M::NFB + f] = Fa[v];
+}
+
 // The far-end layer of the plane at offset `op` (face_of round 2: j-faces
 // past the tile's last row, k-faces past its last column), for the threads
 // of the last wave; j-face f (< NFJ) at (f / TK, f % TK) lies between local
@@ -410,11 +433,12 @@ __device__ __forceinline__ void march_segment(double* lds, const BlockDesc& b, i
       // round 0: the column's low j-face (cells gj-2 .. gj+1 of column gk;
       // in the block wherever the face exists, so its plane offsets are
       // those of the column and the column nk before it)
-      if (fj) march_face<TJ, TK, LIM>(sP, sF, slot - 2 * M::PK, M::PK, rk, c_s - (unsigned)nk, c_s, 3 * nb, nb, t, g);
+      if (fj)
+        march_face_own<TJ, TK, LIM>(sP, sF, slot - 2 * M::PK, M::PK, rk, c_s - (unsigned)nk, 3 * nb, nb, kpvh + 3, t, g);
       __builtin_amdgcn_sched_barrier(0);   // one face at a time: the rounds are not interleaved
       // round 1: the column's low k-face
       if (fk)
-        march_face<TJ, TK, LIM>(sP, sF, slot - 2, 1, rk, c_s - 1u, c_s, 6 * nb, nb, M::NFJ + tj * (TK + 1) + tk, g);
+        march_face_own<TJ, TK, LIM>(sP, sF, slot - 2, 1, rk, c_s - 1u, 6 * nb, nb, kpvh + 6, M::NFJ + tj * (TK + 1) + tk, g);
       __builtin_amdgcn_sched_barrier(0);
       march_far_faces<TJ, TK, LIM>(b, rk, nb, j0, k0, op, sP, sF, g, (int)march_opaque((unsigned)t));
     }

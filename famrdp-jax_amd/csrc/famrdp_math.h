@@ -396,12 +396,11 @@ DF void face_flux_lr_h(const double pL[5], const double pR[5], const double k0h[
 }
 
 
-// One face from the primitives of cells a-1, a, a+1, a+2 and kxyz[d, :] at
-// cells a and a+1 (rhs/invscd.py:53-97), on the stage-kernel forms.
+// The reconstructed states of one face from the primitives of cells a-1, a,
+// a+1, a+2 (rhs/invscd.py:53-97), on the stage-kernel forms.
 template <int LIM>
-DF void face_flux_prim(const double pv0[5], const double pv1[5], const double pv2[5], const double pv3[5],
-                       const double k0[3], const double k1[3], double g, double Fa[5]) {
-  double pL[5], pR[5];
+DF void face_states_prim(const double pv0[5], const double pv1[5], const double pv2[5], const double pv3[5],
+                         double pL[5], double pR[5]) {
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const double sL = limit_fast<LIM>(pv1[v] - pv0[v], pv2[v] - pv1[v]);
@@ -422,7 +421,25 @@ DF void face_flux_prim(const double pv0[5], const double pv1[5], const double pv
   }
   face_state_fence(pL);
   face_state_fence(pR);
+}
+
+// One face from the primitives of cells a-1 .. a+2 and kxyz[d, :] at cells a
+// and a+1.
+template <int LIM>
+DF void face_flux_prim(const double pv0[5], const double pv1[5], const double pv2[5], const double pv3[5],
+                       const double k0[3], const double k1[3], double g, double Fa[5]) {
+  double pL[5], pR[5];
+  face_states_prim<LIM>(pv0, pv1, pv2, pv3, pL, pR);
   face_flux_lr(pL, pR, k0, k1, g, Fa);
+}
+
+// ... from kxyz[d, :] / 2 at cells a and a+1 (face_flux_lr_h: the same bits)
+template <int LIM>
+DF void face_flux_prim_h(const double pv0[5], const double pv1[5], const double pv2[5], const double pv3[5],
+                         const double k0h[3], const double k1h[3], double g, double Fa[5]) {
+  double pL[5], pR[5];
+  face_states_prim<LIM>(pv0, pv1, pv2, pv3, pL, pR);
+  face_flux_lr_h(pL, pR, k0h, k1h, g, Fa);
 }
 
 struct Gas {

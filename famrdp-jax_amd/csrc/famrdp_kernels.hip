@@ -1278,12 +1278,15 @@ namespace famrdp {
   X(9, 4, 8, 8, 320)
 // i-marching kernel: (id, TJ, TK, segment length L, waves per SIMD); its
 // tile table holds (2 x segment index + strip bit, j0, k0) per march_tiling.
+// The 10-plane segments of 18 are for blocks of about 100^3 cells, which the
+// longer segments leave with fewer workgroups than the GPU holds at once.
 #define FAMRDP_MARCH_VARIANTS(X) \
   X(14, 16, 16, 32, 2)           \
   X(15, 8, 32, 20, 2)            \
   X(16, 8, 32, 32, 2)            \
-  X(17, 8, 32, 52, 2)
-constexpr int NUM_TILE_VARIANTS = 18;
+  X(17, 8, 32, 52, 2)            \
+  X(18, 8, 32, 10, 2)
+constexpr int NUM_TILE_VARIANTS = 19;
 
 inline bool is_march_variant(int v) {
   switch (v) {

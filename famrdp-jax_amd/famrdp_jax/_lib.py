@@ -39,11 +39,11 @@ class Bc(ctypes.Structure):
 # (name, argtypes) of every symbol in include/famrdp_capi.h
 # stage-kernel variants of famrdp_plan_autotune / famrdp_plan_set_tile:
 # 0-4 and 10-13 tile shapes of the fused kernel (10-13 and 4 re-stage the
-# primitives), 5-9 of the split pair, 14-17 of the i-marching kernel (the
+# primitives), 5-9 of the split pair, 14-18 of the i-marching kernel (the
 # fused kernel's arithmetic: the same bits)
-NUM_TILE_VARIANTS = 18
-FUSED_VARIANTS = (0, 1, 2, 3, 4, 10, 11, 12, 13, 14, 15, 16, 17)
-MARCH_VARIANTS = (14, 15, 16, 17)
+NUM_TILE_VARIANTS = 19
+FUSED_VARIANTS = (0, 1, 2, 3, 4, 10, 11, 12, 13, 14, 15, 16, 17, 18)
+MARCH_VARIANTS = (14, 15, 16, 17, 18)
 
 SIGNATURES = {
     "famrdp_last_error": ([], ctypes.c_char_p),

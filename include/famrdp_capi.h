@@ -152,7 +152,7 @@ int famrdp_step_stages(famrdp_plan* plan, double* const* q, const double* const*
 /* Stage-kernel selection (no counterpart in the reference).  Variants 0-4
  * and 10-13 are tile shapes of the fused stage kernel (4 and 10-13 stage the
  * primitives twice to need less LDS), 5-9 tile shapes of the split pair
- * (viscous tensor kernel + stage kernel), 14-17 shapes of the i-marching
+ * (viscous tensor kernel + stage kernel), 14-18 shapes of the i-marching
  * kernel (a TJ x TK column walks a segment of TI planes along i; same bits as
  * the fused kernel; blocks under 59,652,323 cells, else set_tile returns
  * FAMRDP_E_INVALID and the autotuner reports 1e30 ms and skips them).  famrdp_plan_autotune times `reps`
@@ -161,7 +161,7 @@ int famrdp_step_stages(famrdp_plan* plan, double* const* q, const double* const*
  * q0 like stage 3), writes the per-variant ms of one pair to ms_out (may be
  * NULL; FAMRDP_NUM_TILE_VARIANTS entries) and keeps the fastest; a variant
  * within 1.5 % of an earlier one counts as a tie and the earlier one stays. */
-#define FAMRDP_NUM_TILE_VARIANTS 18
+#define FAMRDP_NUM_TILE_VARIANTS 19
 int famrdp_plan_autotune(famrdp_plan* plan, double* const* q, const double* const* kxyz, const double* const* vol,
                          double dt, int32_t flags, int32_t reps, double* ms_out, void* stream);
 int famrdp_plan_set_tile(famrdp_plan* plan, int32_t variant);

@@ -197,7 +197,7 @@ int check(int ni, int nj, int nk, int L, Acc& acc, std::vector<int>& cover) {
 extern "C" void hc_march_geom_check(int ni, int nj, int nk, int64_t* out) {
   Acc acc;
   int64_t cells = (int64_t)ni * nj * nk, wrong_cover = 0, strips = 0;
-  const int shapes[4][3] = {{16, 16, 32}, {8, 32, 20}, {8, 32, 32}, {8, 32, 52}};
+  const int shapes[5][3] = {{16, 16, 32}, {8, 32, 20}, {8, 32, 32}, {8, 32, 52}, {8, 32, 10}};
   for (const auto& sh : shapes) {
     std::vector<int> cover((size_t)cells, 0);
     if (sh[0] == 16) strips += check<16, 16>(ni, nj, nk, sh[2], acc, cover);

@@ -43,7 +43,8 @@ def hc(tmp_path_factory):
 # (dims, whether the 8 x 32 launches use strip tiles)
 CASES = [((20, 20, 20), False), ((36, 36, 36), True), ((23, 37, 41), True), ((70, 18, 45), True),
          ((5, 5, 5), False), ((6, 7, 9), False), ((17, 17, 17), False), ((65, 33, 31), False),
-         ((100, 52, 84), True), ((260, 20, 24), False), ((20, 70, 260), True), ((12, 41, 70), True)]
+         ((100, 52, 84), True), ((260, 20, 24), False), ((20, 70, 260), True), ((12, 41, 70), True),
+         ((100, 100, 100), True)]
 
 
 @pytest.mark.parametrize("dims,strips", CASES)

@@ -188,9 +188,10 @@ def _time_adjoint(adj, dt, nsteps, kind, k):
     tm, tm0 = {}, {}
     adj.gradient(dt, nsteps, kind, checkpoint_every=k, timing=tm)
     adj.gradient(dt, nsteps, kind, checkpoint_every=0, timing=tm0)
+    v, dims = adj.plan.tile()
     return dict(steps=nsteps, checkpoint_every=k, forward_ms=fwd * 1e3, gradient_ms=tm["total_s"] * 1e3,
                 ratio=tm["total_s"] / fwd, gradient_all_states_in_memory_ms=tm0["total_s"] * 1e3,
-                ratio_all_states_in_memory=tm0["total_s"] / fwd)
+                ratio_all_states_in_memory=tm0["total_s"] / fwd, forward_variant=v, forward_tile=list(dims))
 
 
 def bench_adjoint():

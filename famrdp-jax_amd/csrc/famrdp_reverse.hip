@@ -37,16 +37,29 @@ __device__ __forceinline__ gmem_double& at(double* a, unsigned i) {
   return *(gmem_double*)(reinterpret_cast<char*>(a) + (i << 3));
 }
 
+__device__ __forceinline__ void rd_prim(const double* __restrict__ q, int64_t N, unsigned c, double g, double pv[5]) {
+  double qc[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qc[v] = rd(q + v * N, c);
+  cons_to_prim_fast(qc, g, pv);
+}
+
 // ---------------------------------------------------------------------------
 // inviscid faces (rhs/invscd.py:53-131 reversed)
 // ---------------------------------------------------------------------------
 template <int LIM>
 __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double* __restrict__ qs,
-                                               const double* __restrict__ lin, double c_mu, unsigned a, double g) {
+                                               const double* __restrict__ lin, double c_mu, unsigned a, double g,
+                                               const double pva[5]) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
   const unsigned njk = (unsigned)nj * (unsigned)nk;
   const int k = (int)(a % (unsigned)nk), j = (int)((a / (unsigned)nk) % (unsigned)nj), i = (int)(a / njk);
+  // the cell's own 1/vol and scaled cotangent, shared by its three faces
+  double lma[5];
+  const double rva = rcp_hd(rd(b.vol, a) + 1e-300);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) lma[v] = c_mu * rd(lin + v * N, a);
   // axes k, j, i: the viscous part that runs first in the same thread ends
   // with the k-neighbours, so the k faces find them in cache; each axis writes
   // only its own face slots, so the order leaves every result unchanged
@@ -60,15 +73,20 @@ __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double*
     if (ad >= 1 && ad <= nd - 3 && (va || vb)) {
       const unsigned a1 = a + sd;
       double lFa[5];
-      const double ra = va ? rcp_hd(rd(b.vol, a) + 1e-300) : 0.0, rb = vb ? rcp_hd(rd(b.vol, a1) + 1e-300) : 0.0;
+      const double ra = va ? rva : 0.0, rb = vb ? rcp_hd(rd(b.vol, a1) + 1e-300) : 0.0;
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double ma = va ? c_mu * rd(lin + v * N, a) : 0.0, mb = vb ? c_mu * rd(lin + v * N, a1) : 0.0;
+        const double ma = va ? lma[v] : 0.0, mb = vb ? c_mu * rd(lin + v * N, a1) : 0.0;
         lFa[v] = -ra * ma + rb * mb;
       }
       double pv[4][5];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
+        if (s == 1) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) pv[1][v] = pva[v];
+          continue;
+        }
         double qq[5];
         const unsigned cc = a + (unsigned)(s - 1) * sd;
 #pragma unroll
@@ -101,16 +119,9 @@ __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double*
 // ---------------------------------------------------------------------------
 // viscous flux tensor (rhs/viscous.py:86-108 and 30-84 reversed)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void rd_prim(const double* __restrict__ q, int64_t N, unsigned c, double g, double pv[5]) {
-  double qc[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) qc[v] = rd(q + v * N, c);
-  cons_to_prim_fast(qc, g, pv);
-}
-
 __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* __restrict__ qs,
                                               const double* __restrict__ lin, double c_mu, unsigned x,
-                                              const Gas& gas) {
+                                              const Gas& gas, const double pv[5]) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
   const unsigned sj = (unsigned)nk, si = (unsigned)nj * (unsigned)nk;
@@ -132,8 +143,7 @@ __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* 
         lG[gidx(r, jj)] += 0.5 * (rd(b.kxyz + (l * 3 + jj) * N, ym) * mm - rd(b.kxyz + (l * 3 + jj) * N, yp) * mp);
     }
   }
-  double pv[5], d[3][4];
-  rd_prim(qs, N, x, gas.gamma, pv);
+  double d[3][4];
   const double T = temperature_fast(pv, gas.R);
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
@@ -279,9 +289,13 @@ k_vjp_faces_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ 
   double* lout;
   vjp_bufs(b, stage, qs, lin, lout);
   const double cmu = stage_cmu(stage, dt * b.dt_scale);
-  cell_vjp_visc(b, qs, lin, cmu, (unsigned)c, gas);
+  // the cell's primitives, converted once for both parts (and rounded once:
+  // the same values reach every use)
+  double pv[5];
+  rd_prim(qs, b.N, (unsigned)c, gas.gamma, pv);
+  cell_vjp_visc(b, qs, lin, cmu, (unsigned)c, gas, pv);
   __builtin_amdgcn_sched_barrier(0);
-  cell_vjp_faces<LIM>(b, qs, lin, cmu, (unsigned)c, gas.gamma);
+  cell_vjp_faces<LIM>(b, qs, lin, cmu, (unsigned)c, gas.gamma, pv);
 }
 
 // at most 128 VGPRs: this memory-bound gather wants four waves per SIMD

@@ -185,6 +185,11 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
   const int64_t N = b.N;
   const unsigned sj = (unsigned)nk, si = (unsigned)nj * (unsigned)nk;
   const int k = (int)(y % sj), j = (int)((y / sj) % (unsigned)nj), i = (int)(y / si);
+  // the cell's own state and primitives, converted once for the three axes
+  double qy[5], pvy[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qy[v] = rd(qs + v * N, y);
+  cons_to_prim_fast(qy, gas.gamma, pvy);
   double lpv[5] = {0, 0, 0, 0, 0};
   // ---- inviscid: face states and MUSCL slopes along each axis
 #pragma unroll
@@ -193,17 +198,23 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
     const int yd = d == 0 ? i : (d == 1 ? j : k);
     const int nd = d == 0 ? ni : (d == 1 ? nj : nk);
     const unsigned sd = d == 0 ? si : (d == 1 ? sj : 1u);
+    double fl[5], fr[5];   // cotangents of the left state of face (y, y+1), right state of (y-1, y)
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      lpv[v] += rd(fsd + v * N, y);                             // left state of face (y, y+1)
-      if (yd >= 1) lpv[v] += rd(fsd + (5 + v) * N, y - sd);     // right state of face (y-1, y)
+      fl[v] = rd(fsd + v * N, y);
+      fr[v] = yd >= 1 ? rd(fsd + (5 + v) * N, y - sd) : 0.0;
+      lpv[v] += fl[v];
+      if (yd >= 1) lpv[v] += fr[v];
     }
     // slopes s(x) = lim(pv(x)-pv(x-1), pv(x+1)-pv(x)) for x = y-1, y, y+1
     double pv[5][5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
       const int xd = yd + s - 2;
-      if (xd >= 0 && xd < nd) rd_prim(qs, N, y + (unsigned)(s - 2) * sd, gas.gamma, pv[s]);
+      if (s == 2)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) pv[2][v] = pvy[v];
+      else if (xd >= 0 && xd < nd) rd_prim(qs, N, y + (unsigned)(s - 2) * sd, gas.gamma, pv[s]);
       else
 #pragma unroll
         for (int v = 0; v < 5; ++v) pv[s][v] = 0.0;
@@ -215,7 +226,8 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
       const unsigned xc = y + (unsigned)o * sd;
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double ls = 0.5 * rd(fsd + v * N, xc) - 0.5 * rd(fsd + (5 + v) * N, xc - sd);
+        const double ls = o == 0 ? 0.5 * fl[v] - 0.5 * fr[v]
+                                 : 0.5 * rd(fsd + v * N, xc) - 0.5 * rd(fsd + (5 + v) * N, xc - sd);
         const double A = pv[o + 2][v] - pv[o + 1][v], B = pv[o + 3][v] - pv[o + 2][v];
         double lA, lB;
         limiter_vjp_t<LIM>(A, B, ls, lA, lB);
@@ -236,10 +248,6 @@ __device__ __forceinline__ void cell_vjp_gather(const BlockDesc& b, const double
 #pragma unroll
     for (int s = 0; s < 4; ++s)
       ls4[s] += 0.5 * rd(b.vs + (4 + 4 * l + s) * N, cm[l]) - 0.5 * rd(b.vs + (4 + 4 * l + s) * N, cp[l]);
-  double qy[5], pvy[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) qy[v] = rd(qs + v * N, y);
-  cons_to_prim_fast(qy, gas.gamma, pvy);
   lpv[1] += ls4[0]; lpv[2] += ls4[1]; lpv[3] += ls4[2];
   temperature_vjp(pvy, gas.R, ls4[3], lpv[0], lpv[4]);
   double gq[5] = {0, 0, 0, 0, 0};

@@ -372,7 +372,7 @@ extern "C" {
 
 const char* famrdp_last_error(void) { return g_last_error.c_str(); }
 
-const char* famrdp_version(void) { return "famrdp-mi355x 0.15 (gfx950, fp64, fused, split and i-marching stage kernels, autotuned)"; }
+const char* famrdp_version(void) { return "famrdp-mi355x 0.16 (gfx950, fp64, fused, split and i-marching stage kernels, autotuned)"; }
 
 int famrdp_cons_to_prim(const double* q, double* pv, int64_t n, double gamma, void* stream) {
   if (n < 0 || (n > 0 && (!q || !pv))) return fail(FAMRDP_E_INVALID, "cons_to_prim: bad arguments");

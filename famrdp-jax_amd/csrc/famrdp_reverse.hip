@@ -50,7 +50,7 @@ __device__ __forceinline__ void rd_prim(const double* __restrict__ q, int64_t N,
 template <int LIM>
 __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double* __restrict__ qs,
                                                const double* __restrict__ lin, double c_mu, unsigned a, double g,
-                                               const double pva[5]) {
+                                               const double pva[5], const double kka[9]) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
   const unsigned njk = (unsigned)nj * (unsigned)nk;
@@ -98,7 +98,7 @@ __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double*
       double k0[3], k1[3];
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        k0[m] = rd(b.kxyz + (d * 3 + m) * N, a);
+        k0[m] = kka[d * 3 + m];
         k1[m] = rd(b.kxyz + (d * 3 + m) * N, a1);
       }
       double nu[3], nm;
@@ -121,7 +121,7 @@ __device__ __forceinline__ void cell_vjp_faces(const BlockDesc& b, const double*
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* __restrict__ qs,
                                               const double* __restrict__ lin, double c_mu, unsigned x,
-                                              const Gas& gas, const double pv[5]) {
+                                              const Gas& gas, const double pv[5], const double kk[9]) {
   const int ni = b.ni, nj = b.nj, nk = b.nk;
   const int64_t N = b.N;
   const unsigned sj = (unsigned)nk, si = (unsigned)nj * (unsigned)nk;
@@ -155,9 +155,6 @@ __device__ __forceinline__ void cell_vjp_visc(const BlockDesc& b, const double* 
     d[l][2] = 0.5 * (pp[3] - pm[3]);
     d[l][3] = 0.5 * (temperature_fast(pp, gas.R) - temperature_fast(pm, gas.R));
   }
-  double kk[9];
-#pragma unroll
-  for (int m = 0; m < 9; ++m) kk[m] = rd(b.kxyz + m * N, x);
   double lu = 0, lv = 0, lw = 0, lT = 0, ld[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
   viscous_flux_vjp(pv, T, d, kk, gas.gamma, gas.R, b.mu0, gas.T0, gas.Ts, gas.Pr_lam, lG, lu, lv, lw, lT, ld);
   at(b.vs, x) = lu;
@@ -298,12 +295,14 @@ k_vjp_faces_visc(const BlockDesc* __restrict__ blocks, const Tile* __restrict__ 
   vjp_bufs(b, stage, qs, lin, lout);
   const double cmu = stage_cmu(stage, dt * b.dt_scale);
   // the cell's primitives, converted once for both parts (and rounded once:
-  // the same values reach every use)
-  double pv[5];
+  // the same values reach every use), and its metrics, loaded once for both
+  double pv[5], kk[9];
   rd_prim(qs, b.N, (unsigned)c, gas.gamma, pv);
-  cell_vjp_visc(b, qs, lin, cmu, (unsigned)c, gas, pv);
+#pragma unroll
+  for (int m = 0; m < 9; ++m) kk[m] = rd(b.kxyz + m * b.N, (unsigned)c);
+  cell_vjp_visc(b, qs, lin, cmu, (unsigned)c, gas, pv, kk);
   __builtin_amdgcn_sched_barrier(0);
-  cell_vjp_faces<LIM>(b, qs, lin, cmu, (unsigned)c, gas.gamma, pv);
+  cell_vjp_faces<LIM>(b, qs, lin, cmu, (unsigned)c, gas.gamma, pv, kk);
 }
 
 // at most 128 VGPRs: this memory-bound gather wants four waves per SIMD
